@@ -1,0 +1,93 @@
+/* voss_amd.h — entry points beyond the reference C ABI.
+ *
+ * These have no counterpart in the reference (library/include/voss/*.h);
+ * they expose what the MI355X engine computes in one pass (checkpointed
+ * Chebyshev tallies, maximal-gap records, the full histogram), batched and
+ * served range queries, GPU Miller-Rabin and Goldbach counts, and the engine
+ * configuration.  All return VOSS_* codes and follow the same conventions as
+ * voss_primes.h (out-parameters written only on success, thread-local last
+ * error).  Implemented by voss_amd/csrc/capi.cpp and serve.cpp.
+ */
+#ifndef VOSS_AMD_H
+#define VOSS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    uint64_t position; /* larger prime of the gap */
+    uint32_t gap;
+    uint32_t pad;
+} voss_amd_gap_t;
+
+typedef struct {
+    uint64_t prime_count;
+    uint64_t pi_4_1, pi_4_3;  /* reference convention: 5 in class 1, 3 in class 3, 2 in neither */
+    uint64_t pi_3_1, pi_3_2;  /* 3 in neither, 2 in class 2: pi_3_1 + pi_3_2 + 1 = pi for N >= 3 */
+    uint64_t last_prime;
+    uint64_t max_gap;
+    uint64_t n_large;         /* gaps >= large_threshold found */
+    uint64_t n_records;       /* maximal-gap records found */
+    uint64_t n_checkpoints;   /* N / cp_interval */
+} voss_amd_full_stats_t;
+
+/* Full statistics over [2, N] in one call.
+ *  hist:        100000 u64 (index = gap; hist[1] = 1 for 2->3), or NULL
+ *  checkpoints: 5 u64 per checkpoint x = j*cp_interval, j = 1..N/cp_interval:
+ *               pi(x), pi(x;4,1), pi(x;4,3), pi(x;3,1), pi(x;3,2); at most cp_cap
+ *               checkpoints written; cp_interval 0 disables; otherwise it must be at
+ *               least 30 x the segment bytes (3.9e6 by default)
+ *  large:       gaps >= large_threshold sorted (gap desc, position asc), at most large_cap
+ *  records:     maximal gaps (each larger than every earlier gap) in ascending
+ *               position, starting (3,1), (5,2); at most rec_cap */
+int voss_amd_full_stats(uint64_t N, uint64_t cp_interval, uint32_t large_threshold,
+                        voss_amd_full_stats_t* out, uint64_t* hist,
+                        uint64_t* checkpoints, uint64_t cp_cap,
+                        voss_amd_gap_t* large, uint64_t large_cap,
+                        voss_amd_gap_t* records, uint64_t rec_cap);
+
+/* Range statistics over the true primes in [a, b] (2, 3, 5 included).
+ * hist[i], i = 1..256, counts consecutive-prime gaps equal to 2i; hist[0] counts
+ * every other gap (1, or above 512).  first/last are 0 when the range holds no prime. */
+typedef struct {
+    uint64_t count;
+    uint64_t first;
+    uint64_t last;
+    uint32_t hist[257];
+    uint32_t pad;
+} voss_amd_range_t;
+
+/* One query, one launch (the plain per-query path). */
+int voss_amd_range_stats(uint64_t a, uint64_t b, voss_amd_range_t* out);
+/* n queries in one persistent launch over the flattened segment work-list. */
+int voss_amd_batch_ranges(const uint64_t* a, const uint64_t* b, uint64_t n, voss_amd_range_t* out);
+/* Prime count in [a, b]. */
+int voss_amd_count_range(uint64_t a, uint64_t b, uint64_t* out);
+
+/* GPU primality of n values (deterministic for every u64); out[i] = 0/1. */
+int voss_amd_is_prime_batch(const uint64_t* x, uint64_t n, uint8_t* out);
+/* GPU next/prev prime of n values (same semantics as voss_primes_next/prev;
+ * prev of x <= 2 gives 0 in out and rc VOSS_OK for the batch). */
+int voss_amd_next_prime_batch(const uint64_t* x, uint64_t n, uint64_t* out);
+int voss_amd_prev_prime_batch(const uint64_t* x, uint64_t n, uint64_t* out);
+/* nth prime without the reference's 5e7 cap (n <= 1e12). */
+int voss_amd_nth_prime(uint64_t n, uint64_t* out);
+/* Goldbach partition counts (p1 <= p2) for even n >= 4, one launch for all. */
+int voss_amd_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out);
+
+/* Engine configuration (the autotuner writes the same fields). */
+int voss_amd_set_config(uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p);
+int voss_amd_get_config(uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p);
+/* Device time of the last engine call (HIP events on the engine stream). */
+int voss_amd_last_kernel_ms(double* sieve_ms, double* stitch_ms, double* total_ms, uint64_t* segments);
+/* Compute units and architecture name of the device in use. */
+int voss_amd_device_info(int* num_cus, char* arch, int arch_len);
+int voss_amd_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOSS_AMD_H */

@@ -1,0 +1,194 @@
+"""GPU parity: the HIP path (through the C ABI / Python package) against the
+CPU oracle on identical inputs, bit for bit.
+
+The oracle itself is pinned by tests/test_oracle.py (reference known answers,
+brute force, invariants)."""
+
+import random
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# N values that exercise segment edges (30 * 128 KiB = 3,932,160 integers per
+# default segment), the p*p rule of low segments and partial last bytes.
+EDGE_N = [7, 8, 10, 29, 30, 31, 100, 101, 1000, 30 * 4096 - 1, 30 * 4096, 3932159, 3932160, 3932161,
+          3932160 * 2 + 17, 10**6, 10**7, 12345677, 10**8]
+
+
+@pytest.mark.parametrize("N", EDGE_N + [2, 3, 4, 5, 6, 10**9])
+def test_prime_count(voss, oracle, N):
+    assert voss.primes.prime_count(N) == oracle.prime_count(N)
+
+
+@pytest.mark.parametrize("N", [10, 100, 10**6, 3932161, 10**7, 10**8, 10**9])
+def test_full_stats_vs_oracle(voss, oracle, N):
+    cp = 10**7 if N >= 10**8 else 0
+    g = voss.stats.full_stats(N, cp_interval=cp)
+    o = oracle.full_stats(N, cp_interval=cp)
+    for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"):
+        assert g[k] == o[k], k
+    assert g["hist"] == o["hist"]
+    assert g["records"] == o["records"]
+    assert g["large_gaps"] == o["large_gaps"]
+    assert g["checkpoints"] == o["checkpoints"]
+
+
+def test_full_stats_large_gap_threshold(voss, oracle):
+    # a low threshold makes the large-gap list long, exercising the append path
+    g = voss.stats.full_stats(10**8, large_threshold=150)
+    o = oracle.full_stats(10**8, large_threshold=150)
+    assert g["large_gaps"] == o["large_gaps"]
+    assert g["n_large"] == len(o["large_gaps"])
+
+
+def test_context_known_answers(voss):
+    with voss.primes.Context(10**9) as c:
+        assert c.prime_count() == 50_847_534
+        assert c.twins() == 3_424_506
+        assert c.cousin() == 3_424_679
+        assert c.sexy() == 6_089_791
+        cb = c.chebyshev()
+        assert (cb.pi_4_1, cb.pi_4_3, cb.difference) == (25_423_491, 25_424_042, 551)
+        assert c.large_gaps() == []
+        s = c.statistics()
+        assert s.total_gaps == 50_847_533
+        assert round(s.mean_gap, 4) == 19.6666
+    with voss.primes.Context(10**6) as c:
+        assert (c.twins(), c.cousin(), c.sexy()) == (8169, 8143, 13549)
+        assert c.statistics().total_gaps == 78497
+        assert c.chebyshev().pi_4_1 == 39175
+    with voss.primes.Context(10**7) as c:
+        s = c.statistics()
+        assert (round(s.mean_gap, 4), round(s.std_dev, 4), round(s.skewness, 4)) == (15.0471, 12.5697, 1.8147)
+
+
+def test_context_minimal_and_progress(voss):
+    seen = []
+    with voss.primes.Context(10**9, profile="minimal") as c:
+        c.set_progress(lambda *a: seen.append(a))
+        assert c.prime_count() == 50_847_534
+    assert seen and seen[-1][1] == len(seen) and seen[-1][3] == 50_847_534
+    assert sum(a[2] for a in seen) == 50_847_534 - 3
+
+
+def test_statistics_match_oracle_moments(voss, oracle):
+    import math
+    o = oracle.full_stats(10**8)
+    h = o["hist"]
+    tot = sum(h[1:])
+    mean = sum(g * h[g] for g in range(1, len(h))) / tot
+    var = sum(h[g] * (g - mean) ** 2 for g in range(1, len(h)) if h[g]) / tot
+    with voss.primes.Context(10**8) as c:
+        s = c.statistics()
+    assert s.total_gaps == tot
+    assert abs(s.mean_gap - mean) < 1e-12 * mean
+    assert abs(s.std_dev - math.sqrt(var)) < 1e-9
+
+
+@pytest.mark.parametrize("a,b", [(2, 10), (10, 30), (100, 130), (10, 100), (0, 1), (2, 2), (90, 110), (2, 1000),
+                                 (10**9, 10**9 + 10**4), (999_999_000, 1_000_001_000), (7, 7), (8, 10),
+                                 (3932150, 3932170)])
+def test_in_range(voss, oracle, a, b):
+    got = voss.primes.in_range(a, b)
+    want = oracle.primes_in_range(max(a, 2), b)
+    if b > 100:
+        want = [p for p in want if p >= 7]  # the reference wheel path omits 2, 3, 5
+    assert got == want
+
+
+def test_in_range_limit(voss):
+    with pytest.raises(voss.VossError):
+        voss.primes.in_range_with_limit(2, 10**6, 100)
+    assert len(voss.primes.in_range_with_limit(1000, 2000, 1000)) == 135
+
+
+def _rand_queries(rng, n, lo_exp, hi_exp, wlo, whi):
+    qs = []
+    for _ in range(n):
+        a = int(10 ** rng.uniform(lo_exp, hi_exp))
+        w = int(10 ** rng.uniform(wlo, whi))
+        qs.append((a, a + w - 1))
+    return qs
+
+
+def test_batch_ranges_vs_oracle(voss, oracle):
+    rng = random.Random(7)
+    qs = _rand_queries(rng, 60, 0, 11, 0, 7.3)
+    qs += [(1, 1), (2, 3), (2, 5), (3, 7), (5, 5), (0, 0), (7, 30), (24, 28), (10**10, 10**10), (4, 4)]
+    got = voss.stats.batch_ranges([q[0] for q in qs], [q[1] for q in qs])
+    for (a, b), g in zip(qs, got):
+        o = oracle.range_stats(a, b)
+        assert (g["count"], g["first"], g["last"]) == (o["count"], o["first"], o["last"]), (a, b)
+        assert g["hist"] == o["hist"], (a, b)
+
+
+def test_batch_equals_plain(voss):
+    rng = random.Random(11)
+    qs = _rand_queries(rng, 40, 6, 12, 3, 8)
+    batch = voss.stats.batch_ranges([q[0] for q in qs], [q[1] for q in qs])
+    for q, bres in zip(qs, batch):
+        assert voss.stats.range_stats(*q) == bres
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 7, 10, 100, 1000, 10**5, 10**6, 10**7, 5 * 10**7])
+def test_nth(voss, oracle, n):
+    assert voss.primes.nth(n) == oracle.nth_prime(n)
+
+
+def test_nth_beyond_reference_cap(voss):
+    assert voss.stats.nth_prime(10**8) == 2_038_074_743
+    assert voss.stats.nth_prime(10**9) == 22_801_763_489
+    with pytest.raises(voss.VossError):
+        voss.primes.nth(50_000_001)
+
+
+def test_goldbach(voss, oracle):
+    ns = [4, 6, 8, 10, 100, 1000, 10**4, 10**5, 123456, 10**6, 999998]
+    assert [voss.primes.goldbach_count(n) for n in ns] == [oracle.goldbach_count_sieve(n) for n in ns]
+    assert voss.stats.goldbach_counts(ns) == oracle.goldbach_counts(ns)
+    assert voss.primes.goldbach_partitions(10) == [(3, 7), (5, 5)]
+    assert voss.primes.goldbach_partitions(20) == [(3, 17), (7, 13)]
+    assert voss.primes.goldbach_partitions(4) == [(2, 2)]
+    r = voss.conjectures.test_goldbach_upto(10**7)
+    assert r["status"] == "no_counterexample_found_upto_N"
+
+
+def test_sophie_germain(voss, oracle):
+    for N, want in [(10, 3), (30, 6), (1000, 37), (10**6, 7746)]:
+        with voss.primes.Context(N) as c:
+            assert c.sophie_germain() == want
+    with voss.primes.Context(10**7 + 3) as c:
+        assert c.sophie_germain() == oracle.sophie_germain(10**7 + 3)
+
+
+def test_mr_batch(voss, oracle):
+    rng = random.Random(3)
+    xs = [0, 1, 2, 3, 4, 5, 2047, 3215031751, 3825123056546413051, 2**64 - 59, 2**64 - 1, 1000000007,
+          18446744073709551557, 4294967297, 4294967291]
+    xs += [rng.getrandbits(64) | 1 for _ in range(500)] + [rng.getrandbits(40) for _ in range(500)]
+    got = voss.stats.is_prime_batch(xs)
+    assert got == [oracle.is_prime(x) for x in xs]
+
+
+def test_next_prev_batch(voss, oracle):
+    rng = random.Random(5)
+    xs = [0, 1, 2, 3, 4, 10, 100, 10**9, 10**12] + [rng.getrandbits(rng.randint(2, 62)) for _ in range(300)]
+    nxt = voss.stats.next_prime_batch(xs)
+    assert nxt == [oracle.next_prime(x)[1] for x in xs]
+    prv = voss.stats.prev_prime_batch(xs)
+    assert prv == [(oracle.prev_prime(x)[1] if x > 2 else 0) for x in xs]
+
+
+@pytest.mark.parametrize("cfg", [(64 * 1024, 1024, 2, 1024), (96 * 1024, 512, 1, 512), (144 * 1024, 1024, 1, 2048)])
+def test_configs_agree(voss, oracle, cfg):
+    old = voss.stats.get_config()
+    try:
+        voss.stats.set_config(*cfg)
+        o = oracle.full_stats(2 * 10**8 + 7, cp_interval=10**7)
+        g = voss.stats.full_stats(2 * 10**8 + 7, cp_interval=10**7)
+        for k in ("prime_count", "pi_4_1", "pi_3_1", "hist", "records", "checkpoints"):
+            assert g[k] == o[k], (cfg, k)
+        assert voss.primes.prime_count(10**9 + 7) == 50847534 + 1  # 10^9+7 is prime
+    finally:
+        voss.stats.set_config(old["seg_bytes"], old["block"], old["wgs_per_cu"], old["tier1_p"])

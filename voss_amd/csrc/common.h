@@ -1,0 +1,98 @@
+// common.h — types shared by the host engine and the gfx950 kernels.
+//
+// Bitmap layout (the reference's, src/voss_w30_v7.cu:38-39): byte k of the
+// global bitmap holds the eight wheel-30 candidates 30k + W30[o], o = 0..7, in
+// bit o.  A segment is a run of consecutive bytes [k0, k0 + nbytes) sieved in
+// one workgroup's LDS.  Values inside a segment are carried as u32 offsets
+// from 30*k0 ("segment-relative values"); 0 means "no prime".
+#pragma once
+#include <stdint.h>
+
+namespace voss {
+
+// Primes 7..19 are removed by copying a periodic pattern into LDS; sieving
+// proper starts at 23.
+constexpr uint32_t kPresievePeriod = 7u * 11u * 13u * 17u * 19u;  // 323,323 bytes
+constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
+constexpr uint32_t kHistBins = 512;   // LDS gap histogram bins (index gap/2)
+constexpr uint32_t kHistCopies = 4;   // copies, one per 16-lane group
+constexpr uint32_t kGapBins = 50000;  // global histogram: index gap/2 for gap < 100000
+constexpr uint32_t kQueryBins = 257;  // per-query histogram: [i] = gap 2i (i=1..256), [0] = other
+constexpr uint32_t kPiTabStep = 1024; // sqrt table granularity
+
+// One large gap (>= threshold).  Same field meaning as the reference's
+// LargeGap (library/src/primes/gaps.cu:25-29): position = larger prime.
+struct GapRec {
+  uint64_t position;
+  uint32_t gap;
+  uint32_t seg;  // segment index (records) or 0
+};
+
+// Per-segment summary written by the sweep kernel.
+struct SegSummary {
+  uint32_t first;   // segment-relative value of the first prime, 0 if none
+  uint32_t last;    // segment-relative value of the last prime, 0 if none
+  uint32_t maxgap;  // largest gap between two primes of this segment, 0 if < 2 primes
+  uint32_t count;   // primes in the segment
+};
+
+// A flagged segment whose records must be listed: every gap > threshold in
+// order, each raising the threshold.
+struct RescanItem {
+  uint32_t seg;
+  uint32_t threshold;
+};
+
+// Batched range query, flattened into segments by the host.
+struct QueryDesc {
+  uint64_t a, b;          // inclusive value range (a >= 7 here; 2, 3, 5 are host seeds)
+  uint64_t k_lo, k_hi;    // byte range [k_lo, k_hi)
+  uint32_t seg_first;     // index of the query's first segment in the work-list
+  uint32_t nseg;
+};
+
+enum KernelMode : int {
+  kModeCount = 0,   // prime count only
+  kModeStats = 1,   // sweep: count, tallies, histogram, summaries, large gaps
+  kModeBatch = 2,   // batched ranges: per-query count, histogram, summaries
+  kModeBitmap = 3,  // sweep: write the sieved, range-masked bytes to a global bitmap
+};
+
+// Arguments of the segment kernel (passed by value).
+struct SieveArgs {
+  // resident tables
+  const uint32_t* primes;       // sieving primes >= 23, bits 24..26 = index of p mod 30
+  const uint32_t* pi_tab;       // pi_tab[i] = #sieving primes <= i*kPiTabStep
+  uint32_t pi_tab_len;
+  const uint32_t* presieve;     // pattern bytes as u32 words, period kPresievePeriod
+  // geometry
+  uint64_t k_base;              // sweep: byte index of segment 0
+  uint64_t k_end;               // sweep: exclusive end byte
+  uint64_t lo_val, hi_val;      // sweep: inclusive value bounds counted
+  uint32_t seg_bytes;           // S (multiple of 16 * waves * 64... see host)
+  uint32_t nseg;                // segments in this launch
+  uint32_t seg_base;            // global index of this launch's segment 0
+  uint32_t tier1_end;           // primes [0, tier1_end): one prime per wave, lanes split the segment
+  uint32_t tier2_end;           // primes [tier1_end, tier2_end): lane = (prime, class); rest: lane = prime
+  unsigned int* work;           // work counter (zeroed before launch)
+  // outputs (mode dependent; unused ones may be null)
+  unsigned long long* totals;   // [0] count, [1] pi_4_1 part, [2] pi_3_1 part
+  SegSummary* summaries;        // indexed by global segment
+  unsigned long long* hist;     // kGapBins u64
+  unsigned long long* buckets;  // per checkpoint bucket: count, c41, c31
+  uint64_t cp_interval;         // 0 = no buckets
+  uint32_t nbuckets;
+  GapRec* large;
+  unsigned int* n_large;
+  uint32_t large_cap;
+  uint32_t large_threshold;
+  // batch mode
+  const QueryDesc* queries;
+  const uint32_t* seg_query;    // query index per segment
+  unsigned int* qhist;          // kQueryBins u32 per query
+  unsigned long long* qcount;   // per query
+  // bitmap mode
+  uint8_t* bitmap;              // byte (k - k_base) of the sieved range
+};
+
+}  // namespace voss

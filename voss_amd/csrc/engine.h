@@ -1,0 +1,155 @@
+// engine.h — host-side orchestration of the gfx950 sieve (one process, one GPU).
+//
+// The Engine owns the device state that stays resident between calls: the
+// sieving-prime table, the sqrt lookup table, the presieve pattern, one
+// stream and reusable scratch buffers.  Every public method throws
+// voss::Error on failure; the C ABI layer maps errors to VOSS_ERR_* codes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace voss {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// Launch configuration chosen by the autotuner (or set by the caller).
+struct TuneConfig {
+  uint32_t seg_bytes = 128 * 1024;  // bytes of bitmap per segment (x30 integers)
+  int block = 1024;                 // threads per workgroup
+  int wgs_per_cu = 1;               // persistent workgroups per CU
+  uint32_t tier1_p = 1024;          // primes below this: one prime per wave
+};
+
+struct GapEntry {
+  uint64_t position;  // larger prime
+  uint32_t gap;
+};
+
+// Result of a full-statistics sweep over [2, N] (N >= 7).
+struct SweepStats {
+  uint64_t prime_count = 0;
+  uint64_t pi_4_1 = 0, pi_4_3 = 0, pi_3_1 = 0, pi_3_2 = 0;
+  uint64_t last_prime = 0;
+  uint64_t max_gap = 0;
+  std::vector<uint64_t> hist;                // index = gap, size 100000
+  std::vector<uint64_t> checkpoints;         // 5 per checkpoint: pi, c41, c43, c31, c32
+  std::vector<GapEntry> large;               // gaps >= threshold, (gap desc, position asc)
+  std::vector<GapEntry> records;             // maximal gaps, ascending position
+  uint64_t large_total = 0;                  // found (may exceed the stored list)
+};
+
+struct RangeOut {
+  uint64_t count = 0, first = 0, last = 0;
+  uint32_t hist[kQueryBins] = {0};
+};
+
+struct KernelTimes {
+  double sieve_ms = 0, stitch_ms = 0, rescan_ms = 0, total_ms = 0;
+  uint64_t segments = 0;
+};
+
+// Device buffer (RAII).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n);
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept;
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+using ProgressFn = std::function<void(int seg, int total, uint64_t in_seg, uint64_t cumulative)>;
+
+class Engine {
+ public:
+  static Engine& get();  // initialises the device on first use; throws if none
+
+  const TuneConfig& config() const { return cfg_; }
+  void set_config(const TuneConfig& c);
+  int num_cus() const { return ncu_; }
+  std::string device_name() const { return name_; }
+  hipStream_t stream() const { return stream_; }
+
+  // pi(N) counting primes <= N (N >= 2).
+  uint64_t prime_count(uint64_t N);
+  // Count primes in [lo, hi] (lo >= 7 handled by the bitmap; smaller lo adds seeds).
+  uint64_t count_range(uint64_t lo, uint64_t hi);
+  // Full statistics over [2, N].
+  SweepStats sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_threshold,
+                         const ProgressFn& progress);
+  // Batched ranges: one persistent launch over the flattened segment list.
+  void batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeOut* out);
+  // All primes in [a, b], ascending (true primes; 2, 3, 5 included).  Sets
+  // *overflow (and returns early) when there are more than max_count.
+  std::vector<uint64_t> primes_in_range(uint64_t a, uint64_t b, uint64_t max_count, bool* overflow);
+  // Sieve [max(lo,7), hi] into a device bitmap (byte k - k_base, bits outside
+  // the range cleared).  Returns the bitmap length in bytes.
+  uint64_t bitmap_range(uint64_t lo, uint64_t hi, DevBuf& out, uint64_t* k_base);
+
+  // Callers around the sieve (engine_callers.hip)
+  std::vector<uint64_t> goldbach_upto(uint64_t N, uint64_t max_ce);       // counterexamples
+  std::vector<uint64_t> goldbach_counts(const uint64_t* ns, size_t k);    // partition counts
+  uint64_t sophie_germain(uint64_t N);                                    // p <= N, 2p+1 prime
+  void is_prime_batch(const uint64_t* x, size_t n, uint8_t* out);
+  void next_prev_batch(const uint64_t* x, size_t n, uint64_t* out, int dir);
+  uint64_t nth_prime(uint64_t n);                                         // 1-indexed
+
+  KernelTimes last_times() const { return times_; }
+
+  // exposed for tests and tuning
+  void ensure_tables(uint64_t max_value);
+  uint32_t tier_index(uint32_t p) const;  // #sieving primes < p
+
+  std::mutex mu;  // one computation at a time per process
+
+ private:
+  Engine();
+  ~Engine();
+  SieveArgs base_args(uint32_t nseg) const;
+  int grid_for(uint32_t nseg) const;
+  uint64_t count_bitmap(uint64_t lo, uint64_t hi);  // wheel-bitmap primes in [max(lo,7), hi]
+  // per-segment prime counts of [7, hi] into summaries (count field), returns #segments
+  uint32_t segment_counts(uint64_t hi, std::vector<uint32_t>& counts);
+
+  // reusable device workspace, grown on demand
+  enum WsSlot {
+    kWsWork, kWsTotals, kWsSummaries, kWsHist, kWsBuckets, kWsLarge, kWsRec, kWsRescan,
+    kWsRescanOut, kWsCounters, kWsQueries, kWsSegQuery, kWsQHist, kWsQCount, kWsList, kWsCount_
+  };
+  DevBuf& ws(int slot, size_t bytes);
+  std::vector<DevBuf> ws_;
+
+  int dev_ = 0, ncu_ = 0;
+  std::string name_;
+  hipStream_t stream_ = nullptr;
+  TuneConfig cfg_;
+  // resident tables
+  std::vector<uint32_t> h_primes_;  // sieving primes >= 23 (plain values)
+  uint32_t* d_primes_ = nullptr;
+  uint32_t* d_pitab_ = nullptr;
+  uint32_t pitab_len_ = 0;
+  uint32_t* d_presieve_ = nullptr;
+  uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
+  KernelTimes times_;
+};
+
+void hip_check(hipError_t e, const char* what);
+// Full statistics for N < 7 (no device needed).
+SweepStats small_sweep_stats(uint64_t N);
+
+}  // namespace voss

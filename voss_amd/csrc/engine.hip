@@ -1,0 +1,633 @@
+// engine.hip — host orchestration of the gfx950 sieve.
+//
+// Host-side flow of a full-statistics sweep over [2, N] (the reference's
+// compute_full_impl, library/src/primes/context.cu:212-559):
+//   1. one persistent launch of the segment kernel over every LDS segment
+//      (in place of one graph of ~27k per-prime launches per 5e10 segment,
+//      an extract, a thrust sort, a mod-4 kernel and a gap kernel);
+//   2. stitch_kernel joins the segments in order (boundary gaps, records);
+//   3. rescan_kernel lists the records inside the few flagged segments;
+//   4. a handful of small device-to-host copies.
+// The seeds 2, 3, 5 and the reference's fix-ups (hist[1] += 1, hist[2] += 1,
+// pi(x;4,1) += 1 for 5, pi(x;4,3) += 1 for 3; context.cu:513-524) are applied
+// here on the host.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "kernels.h"
+
+namespace voss {
+
+namespace {
+const uint32_t kW30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+constexpr int kErrNoDevice = 3, kErrOom = 4, kErrDevice = 5, kErrArg = 6;
+
+uint64_t isqrt64(uint64_t n) {
+  uint64_t r = (uint64_t)std::sqrt((double)n);
+  while (r * r > n) --r;
+  while ((r + 1) * (r + 1) <= n) ++r;
+  return r;
+}
+
+uint32_t env_u32(const char* name, uint32_t dflt) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return (uint32_t)std::strtoul(v, nullptr, 0);
+}
+
+struct Event {
+  hipEvent_t e = nullptr;
+  Event() { hip_check(hipEventCreate(&e), "hipEventCreate"); }
+  ~Event() { if (e) (void)hipEventDestroy(e); }
+};
+float elapsed(const Event& a, const Event& b) {
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, a.e, b.e), "hipEventElapsedTime");
+  return ms;
+}
+}  // namespace
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(kErrDevice, std::string(what) + " failed: " + hipGetErrorString(e));
+}
+
+DevBuf::DevBuf(size_t n) : bytes(n) {
+  if (!n) return;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    throw Error(kErrOom, "hipMalloc(" + std::to_string(n) + " bytes) failed: " + hipGetErrorString(e));
+  }
+}
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    if (p) (void)hipFree(p);
+    p = o.p;
+    bytes = o.bytes;
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  return *this;
+}
+
+Engine& Engine::get() {
+  static Engine* inst = new Engine();  // lives until process exit
+  return *inst;
+}
+
+Engine::Engine() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    throw Error(kErrNoDevice, "no HIP device available");
+  }
+  hip_check(hipGetDevice(&dev_), "hipGetDevice");
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev_), "hipGetDeviceProperties");
+  ncu_ = prop.multiProcessorCount;
+  name_ = prop.gcnArchName;
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+
+  TuneConfig c;
+  c.seg_bytes = env_u32("VOSS_SEG_BYTES", c.seg_bytes);
+  c.block = (int)env_u32("VOSS_BLOCK", (uint32_t)c.block);
+  c.wgs_per_cu = (int)env_u32("VOSS_WGS_PER_CU", (uint32_t)c.wgs_per_cu);
+  c.tier1_p = env_u32("VOSS_TIER1_P", c.tier1_p);
+  set_config(c);
+
+  // presieve pattern: byte j has bit o set iff 30j + W30[o] has no factor in 7..19
+  const size_t pat_bytes = ((size_t)kPresievePeriod + kMaxSegBytes + 4096) & ~(size_t)3;
+  std::vector<uint8_t> pat(pat_bytes);
+  for (size_t j = 0; j < pat_bytes; ++j) {
+    uint8_t b = 0;
+    for (int o = 0; o < 8; ++o) {
+      uint64_t v = 30ull * j + kW30[o];
+      if (v % 7 && v % 11 && v % 13 && v % 17 && v % 19) b |= (uint8_t)(1u << o);
+    }
+    pat[j] = b;
+  }
+  hip_check(hipMalloc(&d_presieve_, pat_bytes), "hipMalloc(presieve)");
+  hip_check(hipMemcpy(d_presieve_, pat.data(), pat_bytes, hipMemcpyHostToDevice), "hipMemcpy(presieve)");
+  hip_check(prepare_kernels(lds_bytes_for(kMaxSegBytes)), "prepare_kernels");
+  ensure_tables(1000000000000ull);
+}
+
+Engine::~Engine() {}
+
+void Engine::set_config(const TuneConfig& c) {
+  if (c.block != 512 && c.block != 1024) throw Error(kErrArg, "block must be 512 or 1024");
+  const uint32_t nw = (uint32_t)c.block / 64;
+  if (c.seg_bytes == 0 || c.seg_bytes > kMaxSegBytes || c.seg_bytes % (1024 * nw) != 0)
+    throw Error(kErrArg, "seg_bytes must be a multiple of 1 KiB x waves and at most 144 KiB");
+  if (c.wgs_per_cu < 1 || c.wgs_per_cu > 8) throw Error(kErrArg, "wgs_per_cu must be in [1, 8]");
+  if ((uint64_t)lds_bytes_for(c.seg_bytes) * (uint64_t)c.wgs_per_cu > 160 * 1024)
+    throw Error(kErrArg, "segment LDS x workgroups per CU exceeds 160 KiB");
+  cfg_ = c;
+}
+
+void Engine::ensure_tables(uint64_t max_value) {
+  const uint64_t need = isqrt64(max_value) + 2;
+  if (need <= table_limit_) return;
+  uint64_t lim = std::max<uint64_t>(need + need / 4, 1u << 16);
+  if (lim > (1ull << 24) - 1) lim = (1ull << 24) - 1;  // 1.67e7 > sqrt(1e14)
+  std::vector<uint8_t> s(lim + 1, 1);
+  s[0] = s[1] = 0;
+  for (uint64_t i = 2; i * i <= lim; ++i)
+    if (s[i]) for (uint64_t j = i * i; j <= lim; j += i) s[j] = 0;
+  std::vector<uint32_t> entries;
+  h_primes_.clear();
+  for (uint64_t p = 23; p <= lim; ++p) {
+    if (!s[p]) continue;
+    uint32_t ridx = 0;
+    for (uint32_t o = 0; o < 8; ++o) if (kW30[o] == p % 30) ridx = o;
+    h_primes_.push_back((uint32_t)p);
+    entries.push_back((uint32_t)p | (ridx << 24));
+  }
+  const uint32_t len = (uint32_t)(lim / kPiTabStep + 2);
+  std::vector<uint32_t> pitab(len);
+  size_t k = 0;
+  for (uint32_t i = 0; i < len; ++i) {
+    const uint64_t bound = (uint64_t)i * kPiTabStep;
+    while (k < h_primes_.size() && h_primes_[k] <= bound) ++k;
+    pitab[i] = (uint32_t)k;
+  }
+  if (d_primes_) (void)hipFree(d_primes_);
+  if (d_pitab_) (void)hipFree(d_pitab_);
+  d_primes_ = nullptr;
+  d_pitab_ = nullptr;
+  hip_check(hipMalloc(&d_primes_, entries.size() * 4 + 4), "hipMalloc(primes)");
+  hip_check(hipMalloc(&d_pitab_, (size_t)len * 4), "hipMalloc(pi_tab)");
+  hip_check(hipMemcpy(d_primes_, entries.data(), entries.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(primes)");
+  hip_check(hipMemcpy(d_pitab_, pitab.data(), (size_t)len * 4, hipMemcpyHostToDevice), "hipMemcpy(pi_tab)");
+  pitab_len_ = len;
+  table_limit_ = lim;
+}
+
+uint32_t Engine::tier_index(uint32_t p) const {
+  return (uint32_t)(std::lower_bound(h_primes_.begin(), h_primes_.end(), p) - h_primes_.begin());
+}
+
+SieveArgs Engine::base_args(uint32_t nseg) const {
+  SieveArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.primes = d_primes_;
+  A.pi_tab = d_pitab_;
+  A.pi_tab_len = pitab_len_;
+  A.presieve = d_presieve_;
+  A.seg_bytes = cfg_.seg_bytes;
+  A.nseg = nseg;
+  A.tier1_end = tier_index(cfg_.tier1_p);
+  A.tier2_end = tier_index(cfg_.seg_bytes);
+  A.large_threshold = 0xFFFFFFFFu;
+  return A;
+}
+
+int Engine::grid_for(uint32_t nseg) const {
+  uint64_t g = (uint64_t)ncu_ * (uint64_t)cfg_.wgs_per_cu;
+  if (g > nseg) g = nseg;
+  return (int)std::max<uint64_t>(g, 1);
+}
+
+DevBuf& Engine::ws(int slot, size_t bytes) {
+  if (slot >= (int)ws_.size()) ws_.resize(slot + 1);
+  if (ws_[slot].bytes < bytes) {
+    ws_[slot] = DevBuf();
+    ws_[slot] = DevBuf(std::max(bytes, (size_t)256));
+  }
+  return ws_[slot];
+}
+
+// ------------------------------------------------------------------ counting
+uint64_t Engine::count_bitmap(uint64_t lo, uint64_t hi) {
+  if (lo < 7) lo = 7;
+  if (hi < lo) return 0;
+  ensure_tables(hi);
+  const uint32_t S = cfg_.seg_bytes;
+  const uint64_t k_base = lo / 30, k_end = hi / 30 + 1;
+  const uint64_t nseg64 = (k_end - k_base + S - 1) / S;
+  if (nseg64 > 0xFFFFFFF0ull) throw Error(kErrArg, "range too large");
+  const uint32_t nseg = (uint32_t)nseg64;
+  DevBuf& wk = ws(kWsWork, 64);
+  DevBuf& tot = ws(kWsTotals, 64);
+  hip_check(hipMemsetAsync(wk.p, 0, 64, stream_), "memset");
+  hip_check(hipMemsetAsync(tot.p, 0, 64, stream_), "memset");
+  SieveArgs A = base_args(nseg);
+  A.k_base = k_base;
+  A.k_end = k_end;
+  A.lo_val = lo;
+  A.hi_val = hi;
+  A.work = wk.as<unsigned int>();
+  A.totals = tot.as<unsigned long long>();
+  Event e0, e1;
+  hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+  hip_check(launch_sieve(kModeCount, cfg_.block, A, grid_for(nseg), stream_), "launch(count)");
+  hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+  unsigned long long c = 0;
+  hip_check(hipMemcpyAsync(&c, tot.p, 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy(count)");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  times_ = KernelTimes();
+  times_.sieve_ms = elapsed(e0, e1);
+  times_.total_ms = times_.sieve_ms;
+  times_.segments = nseg;
+  return c;
+}
+
+uint64_t Engine::count_range(uint64_t lo, uint64_t hi) {
+  if (hi < lo) return 0;
+  uint64_t c = 0;
+  for (uint64_t sd : {2ull, 3ull, 5ull}) c += (sd >= lo && sd <= hi) ? 1 : 0;
+  return c + count_bitmap(lo, hi);
+}
+
+uint64_t Engine::prime_count(uint64_t N) {
+  if (N < 2) return 0;
+  if (N < 7) return N < 3 ? 1 : (N < 5 ? 2 : 3);
+  return 3 + count_bitmap(7, N);
+}
+
+// ------------------------------------------------------------------ full statistics
+// N < 7: the reference short-circuits (context.cu:228-247) with pi = 1/2/3,
+// hist[1] / hist[2] set and the mod-4 tallies left at 0.
+SweepStats small_sweep_stats(uint64_t N) {
+  SweepStats R;
+  R.hist.assign(100000, 0);
+  const uint64_t seeds[3] = {2, 3, 5};
+  uint64_t nseed = 0;
+  for (uint64_t sd : seeds) nseed += sd <= N ? 1 : 0;
+  R.prime_count = nseed;
+  if (N >= 3) { R.hist[1] = 1; R.records.push_back({3, 1}); }
+  if (N >= 5) { R.hist[2] = 1; R.records.push_back({5, 2}); }
+  R.last_prime = nseed ? seeds[nseed - 1] : 0;
+  R.max_gap = nseed >= 3 ? 2 : (nseed == 2 ? 1 : 0);
+  R.pi_3_2 = N >= 5 ? 2 : (N >= 2 ? 1 : 0);
+  return R;
+}
+
+SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_threshold,
+                               const ProgressFn& progress) {
+  SweepStats R;
+  R.hist.assign(100000, 0);
+  if (N < 2) throw Error(1, "N must be >= 2");
+  const uint32_t S = cfg_.seg_bytes;
+  if (cp_interval && cp_interval < 30ull * S)
+    throw Error(kErrArg, "checkpoint interval must be at least " + std::to_string(30ull * S));
+  const uint64_t ncp = cp_interval ? N / cp_interval : 0;
+  const uint64_t nbuckets = cp_interval ? (N + cp_interval - 1) / cp_interval : 0;
+  if (N < 7) return small_sweep_stats(N);
+  const uint64_t nseed = 3;
+  R.records.push_back({3, 1});
+  R.records.push_back({5, 2});
+  ensure_tables(N);
+  const uint64_t k_end = N / 30 + 1;
+  const uint64_t nseg64 = (k_end + S - 1) / S;
+  if (nseg64 > 0x7FFFFFF0ull) throw Error(kErrArg, "N too large for the segment size");
+  const uint32_t nseg = (uint32_t)nseg64;
+  const uint32_t large_cap = 1u << 20, rec_cap = 4096, rescan_cap = 4096, rout_cap = 16384;
+
+  // progress: one launch per reference-sized segment (5e10+10 integers,
+  // context.cu:260-269), otherwise a single launch
+  uint32_t per_launch = nseg;
+  if (progress) {
+    const uint64_t ref_seg = 50000000010ull;
+    per_launch = (uint32_t)std::max<uint64_t>(1, ref_seg / (30ull * S));
+  }
+  const uint32_t nlaunch = (nseg + per_launch - 1) / per_launch;
+
+  DevBuf& d_sum = ws(kWsSummaries, (size_t)nseg * sizeof(SegSummary));
+  DevBuf& d_tot = ws(kWsTotals, 64);
+  DevBuf& d_hist = ws(kWsHist, (size_t)kGapBins * 8);
+  DevBuf& d_bk = ws(kWsBuckets, (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8);
+  DevBuf& d_large = ws(kWsLarge, (size_t)large_cap * sizeof(GapRec));
+  DevBuf& d_rec = ws(kWsRec, (size_t)rec_cap * sizeof(GapRec));
+  DevBuf& d_resc = ws(kWsRescan, (size_t)rescan_cap * sizeof(RescanItem));
+  DevBuf& d_rout = ws(kWsRescanOut, (size_t)rout_cap * sizeof(GapRec));
+  DevBuf& d_cnt = ws(kWsCounters, 64);  // [0] n_large [1] n_rec [2] n_rescan [3] n_rout, [8..] u64 final
+  DevBuf& d_work = ws(kWsWork, (size_t)std::max<uint32_t>(nlaunch, 1) * 4 + 64);
+
+  hip_check(hipMemsetAsync(d_tot.p, 0, 64, stream_), "memset");
+  hip_check(hipMemsetAsync(d_hist.p, 0, (size_t)kGapBins * 8, stream_), "memset");
+  hip_check(hipMemsetAsync(d_bk.p, 0, (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8, stream_), "memset");
+  hip_check(hipMemsetAsync(d_cnt.p, 0, 64, stream_), "memset");
+  hip_check(hipMemsetAsync(d_work.p, 0, (size_t)std::max<uint32_t>(nlaunch, 1) * 4 + 64, stream_), "memset");
+
+  SieveArgs A = base_args(nseg);
+  A.k_base = 0;
+  A.k_end = k_end;
+  A.lo_val = 7;
+  A.hi_val = N;
+  A.totals = d_tot.as<unsigned long long>();
+  A.summaries = d_sum.as<SegSummary>();
+  A.hist = d_hist.as<unsigned long long>();
+  A.buckets = d_bk.as<unsigned long long>();
+  A.cp_interval = cp_interval;
+  A.nbuckets = (uint32_t)nbuckets;
+  A.large = d_large.as<GapRec>();
+  A.n_large = d_cnt.as<unsigned int>() + 0;
+  A.large_cap = large_cap;
+  A.large_threshold = large_threshold;
+
+  Event e0, e1, e2, e3;
+  hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+  uint64_t cumulative = nseed;
+  for (uint32_t L = 0; L < nlaunch; ++L) {
+    SieveArgs AL = A;
+    AL.seg_base = L * per_launch;
+    AL.nseg = std::min(per_launch, nseg - AL.seg_base);
+    AL.work = d_work.as<unsigned int>() + L;
+    hip_check(launch_sieve(kModeStats, cfg_.block, AL, grid_for(AL.nseg), stream_), "launch(stats)");
+    if (progress) {
+      unsigned long long c = 0;
+      hip_check(hipMemcpyAsync(&c, d_tot.p, 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy(progress)");
+      hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+      const uint64_t now = nseed + c;
+      progress((int)L + 1, (int)nlaunch, now - cumulative, now);
+      cumulative = now;
+    }
+  }
+  hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+
+  StitchArgs T;
+  std::memset(&T, 0, sizeof(T));
+  T.summaries = d_sum.as<SegSummary>();
+  T.nseg = nseg;
+  T.seg_bytes = S;
+  T.k_base = 0;
+  T.seed_last = 5;
+  T.seed_max = 2;
+  T.large_threshold = large_threshold;
+  T.hist = d_hist.as<unsigned long long>();
+  T.large = d_large.as<GapRec>();
+  T.n_large = d_cnt.as<unsigned int>() + 0;
+  T.large_cap = large_cap;
+  T.rec = d_rec.as<GapRec>();
+  T.n_rec = d_cnt.as<unsigned int>() + 1;
+  T.rec_cap = rec_cap;
+  T.rescan = d_resc.as<RescanItem>();
+  T.n_rescan = d_cnt.as<unsigned int>() + 2;
+  T.rescan_cap = rescan_cap;
+  T.final_out = d_cnt.as<unsigned long long>() + 4;
+  hip_check(launch_stitch(T, stream_), "launch(stitch)");
+  hip_check(hipEventRecord(e2.e, stream_), "hipEventRecord");
+
+  unsigned int cnts[4];
+  unsigned long long fin[2];
+  unsigned long long tot[3];
+  hip_check(hipMemcpyAsync(cnts, d_cnt.p, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipMemcpyAsync(fin, d_cnt.as<unsigned long long>() + 4, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipMemcpyAsync(tot, d_tot.p, 24, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  const uint32_t n_resc = cnts[2];
+  if (n_resc > rescan_cap) throw Error(7, "record rescan list overflow");
+  if (n_resc) {
+    RescanArgs RA;
+    RA.items = d_resc.as<RescanItem>();
+    RA.out = d_rout.as<GapRec>();
+    RA.n_out = d_cnt.as<unsigned int>() + 3;
+    RA.out_cap = rout_cap;
+    hip_check(launch_rescan(cfg_.block, A, RA, (int)n_resc, stream_), "launch(rescan)");
+  }
+  hip_check(hipEventRecord(e3.e, stream_), "hipEventRecord");
+
+  std::vector<unsigned long long> dh(kGapBins);
+  std::vector<unsigned long long> bk(std::max<uint64_t>(nbuckets, 1) * 3);
+  hip_check(hipMemcpyAsync(dh.data(), d_hist.p, (size_t)kGapBins * 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  if (nbuckets)
+    hip_check(hipMemcpyAsync(bk.data(), d_bk.p, (size_t)nbuckets * 3 * 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipMemcpyAsync(cnts, d_cnt.p, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  const uint32_t n_large = std::min(cnts[0], large_cap), n_rec = std::min(cnts[1], rec_cap),
+                 n_rout = std::min(cnts[3], rout_cap);
+  std::vector<GapRec> lg(n_large), rc(n_rec), ro(n_rout);
+  if (n_large) hip_check(hipMemcpyAsync(lg.data(), d_large.p, n_large * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  if (n_rec) hip_check(hipMemcpyAsync(rc.data(), d_rec.p, n_rec * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  if (n_rout) hip_check(hipMemcpyAsync(ro.data(), d_rout.p, n_rout * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+
+  times_ = KernelTimes();
+  times_.sieve_ms = elapsed(e0, e1);
+  times_.stitch_ms = elapsed(e1, e2);
+  times_.rescan_ms = elapsed(e2, e3);
+  times_.total_ms = elapsed(e0, e3);
+  times_.segments = nseg;
+
+  // ---- assemble (reference conventions)
+  const uint64_t cnt = tot[0];
+  R.prime_count = nseed + cnt;
+  for (uint32_t h = 1; h < kGapBins; ++h) R.hist[2 * h] += dh[h];
+  R.hist[1] += 1;  // 2 -> 3
+  R.hist[2] += 1;  // 3 -> 5
+  R.pi_4_1 = tot[1] + 1;
+  R.pi_4_3 = cnt - tot[1] + 1;
+  R.pi_3_1 = tot[2];
+  R.pi_3_2 = cnt - tot[2] + 2;
+  R.last_prime = fin[0];
+  R.max_gap = fin[1];
+  if (ncp) {
+    R.checkpoints.resize(5 * ncp);
+    uint64_t c = 0, c41 = 0, c31 = 0;
+    for (uint64_t j = 0; j < ncp; ++j) {
+      c += bk[3 * j]; c41 += bk[3 * j + 1]; c31 += bk[3 * j + 2];
+      R.checkpoints[5 * j + 0] = c + 3;
+      R.checkpoints[5 * j + 1] = c41 + 1;
+      R.checkpoints[5 * j + 2] = c - c41 + 1;
+      R.checkpoints[5 * j + 3] = c31;
+      R.checkpoints[5 * j + 4] = c - c31 + 2;
+    }
+  }
+  R.large_total = cnts[0];
+  R.large.reserve(lg.size());
+  for (const GapRec& g : lg) R.large.push_back({g.position, g.gap});
+  std::sort(R.large.begin(), R.large.end(), [](const GapEntry& x, const GapEntry& y) {
+    return x.gap != y.gap ? x.gap > y.gap : x.position < y.position;
+  });
+  std::vector<GapEntry> recs;
+  for (const GapRec& g : rc) recs.push_back({g.position, g.gap});
+  for (const GapRec& g : ro) recs.push_back({g.position, g.gap});
+  std::sort(recs.begin(), recs.end(), [](const GapEntry& x, const GapEntry& y) { return x.position < y.position; });
+  for (const GapEntry& g : recs) R.records.push_back(g);
+  return R;
+}
+
+// ------------------------------------------------------------------ batched ranges
+void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeOut* out) {
+  const uint32_t S = cfg_.seg_bytes;
+  std::vector<QueryDesc> qs;
+  std::vector<uint32_t> qmap;  // device query -> caller index
+  uint64_t bmax = 7;
+  uint64_t total = 0;
+  for (size_t i = 0; i < n; ++i) {
+    out[i] = RangeOut();
+    const uint64_t lo = std::max<uint64_t>(a[i], 7);
+    if (b[i] < lo) continue;
+    QueryDesc Q;
+    Q.a = lo;
+    Q.b = b[i];
+    Q.k_lo = lo / 30;
+    Q.k_hi = b[i] / 30 + 1;
+    const uint64_t ns = (Q.k_hi - Q.k_lo + S - 1) / S;
+    Q.seg_first = (uint32_t)total;
+    Q.nseg = (uint32_t)ns;
+    total += ns;
+    if (total > 0x7FFFFFF0ull) throw Error(kErrArg, "batch too large");
+    bmax = std::max(bmax, b[i]);
+    qs.push_back(Q);
+    qmap.push_back((uint32_t)i);
+  }
+  const uint32_t nq = (uint32_t)qs.size();
+  const uint32_t nseg = (uint32_t)total;
+  times_ = KernelTimes();
+  if (nq) {
+    ensure_tables(bmax);
+    std::vector<uint32_t> segq(nseg);
+    for (uint32_t q = 0; q < nq; ++q)
+      for (uint32_t i = 0; i < qs[q].nseg; ++i) segq[qs[q].seg_first + i] = q;
+    DevBuf& d_q = ws(kWsQueries, (size_t)nq * sizeof(QueryDesc));
+    DevBuf& d_sq = ws(kWsSegQuery, (size_t)nseg * 4);
+    DevBuf& d_sum = ws(kWsSummaries, (size_t)nseg * sizeof(SegSummary));
+    DevBuf& d_qh = ws(kWsQHist, (size_t)nq * kQueryBins * 4);
+    DevBuf& d_qc = ws(kWsQCount, (size_t)nq * 8 * 3);
+    DevBuf& d_work = ws(kWsWork, 64);
+    hip_check(hipMemcpyAsync(d_q.p, qs.data(), (size_t)nq * sizeof(QueryDesc), hipMemcpyHostToDevice, stream_), "H2D");
+    hip_check(hipMemcpyAsync(d_sq.p, segq.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_), "H2D");
+    hip_check(hipMemsetAsync(d_qh.p, 0, (size_t)nq * kQueryBins * 4, stream_), "memset");
+    hip_check(hipMemsetAsync(d_qc.p, 0, (size_t)nq * 8 * 3, stream_), "memset");
+    hip_check(hipMemsetAsync(d_work.p, 0, 64, stream_), "memset");
+    SieveArgs A = base_args(nseg);
+    A.work = d_work.as<unsigned int>();
+    A.summaries = d_sum.as<SegSummary>();
+    A.queries = d_q.as<QueryDesc>();
+    A.seg_query = d_sq.as<uint32_t>();
+    A.qhist = d_qh.as<unsigned int>();
+    A.qcount = d_qc.as<unsigned long long>();
+    BatchStitchArgs B;
+    B.queries = d_q.as<QueryDesc>();
+    B.summaries = d_sum.as<SegSummary>();
+    B.nq = nq;
+    B.seg_bytes = S;
+    B.qhist = d_qh.as<unsigned int>();
+    B.first = d_qc.as<unsigned long long>() + nq;
+    B.last = d_qc.as<unsigned long long>() + 2 * nq;
+    Event e0, e1, e2;
+    hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+    hip_check(launch_sieve(kModeBatch, cfg_.block, A, grid_for(nseg), stream_), "launch(batch)");
+    hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+    hip_check(launch_batch_stitch(B, stream_), "launch(batch_stitch)");
+    hip_check(hipEventRecord(e2.e, stream_), "hipEventRecord");
+    std::vector<unsigned long long> qc((size_t)nq * 3);
+    std::vector<uint32_t> qh((size_t)nq * kQueryBins);
+    hip_check(hipMemcpyAsync(qc.data(), d_qc.p, (size_t)nq * 24, hipMemcpyDeviceToHost, stream_), "D2H");
+    hip_check(hipMemcpyAsync(qh.data(), d_qh.p, (size_t)nq * kQueryBins * 4, hipMemcpyDeviceToHost, stream_), "D2H");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    times_.sieve_ms = elapsed(e0, e1);
+    times_.stitch_ms = elapsed(e1, e2);
+    times_.total_ms = elapsed(e0, e2);
+    times_.segments = nseg;
+    for (uint32_t q = 0; q < nq; ++q) {
+      RangeOut& o = out[qmap[q]];
+      o.count = qc[q];
+      o.first = qc[nq + q];
+      o.last = qc[2 * nq + q];
+      std::memcpy(o.hist, qh.data() + (size_t)q * kQueryBins, kQueryBins * 4);
+    }
+  }
+  // seeds 2, 3, 5 (not in the wheel bitmap) for queries that include them
+  for (size_t i = 0; i < n; ++i) {
+    if (a[i] > 5 || b[i] < 2 || b[i] < a[i]) continue;
+    RangeOut& o = out[i];
+    std::vector<uint64_t> sd;
+    for (uint64_t s : {2ull, 3ull, 5ull}) if (s >= a[i] && s <= b[i]) sd.push_back(s);
+    if (sd.empty()) continue;
+    auto add_gap = [&](uint64_t g) {
+      if (g >= 2 && g <= 512 && !(g & 1)) o.hist[g / 2]++; else o.hist[0]++;
+    };
+    for (size_t k = 1; k < sd.size(); ++k) add_gap(sd[k] - sd[k - 1]);
+    if (o.count) add_gap(o.first - sd.back());
+    else o.last = sd.back();
+    o.first = sd.front();
+    o.count += sd.size();
+  }
+}
+
+// ------------------------------------------------------------------ bitmap + list
+uint64_t Engine::bitmap_range(uint64_t lo, uint64_t hi, DevBuf& out, uint64_t* k_base_out) {
+  if (lo < 7) lo = 7;
+  const uint32_t S = cfg_.seg_bytes;
+  const uint64_t k_base = lo / 30, k_end = hi / 30 + 1;
+  *k_base_out = k_base;
+  if (hi < lo) return 0;
+  ensure_tables(hi);
+  const uint64_t nbytes = k_end - k_base;
+  const uint64_t nseg64 = (nbytes + S - 1) / S;
+  if (nseg64 > 0x7FFFFFF0ull) throw Error(kErrArg, "range too large");
+  const uint32_t nseg = (uint32_t)nseg64;
+  if (out.bytes < nseg64 * S + 64) out = DevBuf(nseg64 * S + 64);
+  DevBuf& wk = ws(kWsWork, 64);
+  hip_check(hipMemsetAsync(wk.p, 0, 64, stream_), "memset");
+  SieveArgs A = base_args(nseg);
+  A.k_base = k_base;
+  A.k_end = k_end;
+  A.lo_val = lo;
+  A.hi_val = hi;
+  A.work = wk.as<unsigned int>();
+  A.bitmap = out.as<uint8_t>();
+  Event e0, e1;
+  hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+  hip_check(launch_sieve(kModeBitmap, cfg_.block, A, grid_for(nseg), stream_), "launch(bitmap)");
+  hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  times_ = KernelTimes();
+  times_.sieve_ms = elapsed(e0, e1);
+  times_.total_ms = times_.sieve_ms;
+  times_.segments = nseg;
+  return nbytes;
+}
+
+std::vector<uint64_t> Engine::primes_in_range(uint64_t a, uint64_t b, uint64_t max_count, bool* overflow) {
+  std::vector<uint64_t> res;
+  *overflow = false;
+  for (uint64_t sd : {2ull, 3ull, 5ull})
+    if (sd >= a && sd <= b) res.push_back(sd);
+  const uint64_t lo = std::max<uint64_t>(a, 7);
+  if (b < lo) {
+    if (res.size() > max_count) *overflow = true;
+    return res;
+  }
+  DevBuf bm;
+  uint64_t k_base = 0;
+  const uint64_t nbytes = bitmap_range(lo, b, bm, &k_base);
+  const uint64_t nblk = (nbytes + kListBlockBytes - 1) / kListBlockBytes;
+  DevBuf d_cnt(nblk * 4 + 4), d_off(nblk * 8 + 8);
+  hip_check(launch_list_count(bm.as<uint8_t>(), nbytes, d_cnt.as<uint32_t>(), stream_), "launch(list_count)");
+  std::vector<uint32_t> cnt(nblk);
+  hip_check(hipMemcpyAsync(cnt.data(), d_cnt.p, nblk * 4, hipMemcpyDeviceToHost, stream_), "D2H");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  std::vector<unsigned long long> off(nblk);
+  uint64_t total = 0;
+  for (uint64_t i = 0; i < nblk; ++i) { off[i] = total; total += cnt[i]; }
+  if (res.size() + total > max_count) {
+    *overflow = true;
+    return res;
+  }
+  if (total) {
+    DevBuf d_out(total * 8);
+    hip_check(hipMemcpyAsync(d_off.p, off.data(), nblk * 8, hipMemcpyHostToDevice, stream_), "H2D");
+    hip_check(launch_list_write(bm.as<uint8_t>(), nbytes, k_base, d_off.as<unsigned long long>(),
+                                d_out.as<uint64_t>(), stream_), "launch(list_write)");
+    const size_t s0 = res.size();
+    res.resize(s0 + total);
+    hip_check(hipMemcpyAsync(res.data() + s0, d_out.p, total * 8, hipMemcpyDeviceToHost, stream_), "D2H");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  }
+  return res;
+}
+
+}  // namespace voss

@@ -1,0 +1,65 @@
+// kernels.h — launch entry points of the gfx950 kernels (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace voss {
+
+struct StitchArgs {
+  const SegSummary* summaries;
+  uint32_t nseg;
+  uint32_t seg_bytes;
+  uint64_t k_base;
+  uint64_t seed_last;   // last prime before the first segment (5)
+  uint32_t seed_max;    // largest gap among the seeds (2)
+  uint32_t large_threshold;
+  unsigned long long* hist;
+  GapRec* large;
+  unsigned int* n_large;
+  uint32_t large_cap;
+  GapRec* rec;
+  unsigned int* n_rec;
+  uint32_t rec_cap;
+  RescanItem* rescan;
+  unsigned int* n_rescan;
+  uint32_t rescan_cap;
+  unsigned long long* final_out;  // [0] last prime, [1] largest gap
+};
+
+struct RescanArgs {
+  const RescanItem* items;
+  GapRec* out;
+  unsigned int* n_out;
+  uint32_t out_cap;
+};
+
+struct BatchStitchArgs {
+  const QueryDesc* queries;
+  const SegSummary* summaries;
+  uint32_t nq;
+  uint32_t seg_bytes;
+  unsigned int* qhist;
+  unsigned long long* first;
+  unsigned long long* last;
+};
+
+// Dynamic LDS bytes the segment kernels need for a segment of seg_bytes.
+uint32_t lds_bytes_for(uint32_t seg_bytes);
+
+// Raise the dynamic-LDS limit of every segment kernel to lds_bytes.
+hipError_t prepare_kernels(uint32_t lds_bytes);
+
+hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipStream_t st);
+hipError_t launch_stitch(const StitchArgs& A, hipStream_t st);
+hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st);
+hipError_t launch_batch_stitch(const BatchStitchArgs& A, hipStream_t st);
+
+// Bitmap -> ascending list of primes, in blocks of 4096 bitmap bytes:
+// counts[b] = primes in block b; out[block_off[b] + i] = i-th prime of block b.
+constexpr uint32_t kListBlockBytes = 4096;
+hipError_t launch_list_count(const uint8_t* bm, uint64_t nbytes, uint32_t* counts, hipStream_t st);
+hipError_t launch_list_write(const uint8_t* bm, uint64_t nbytes, uint64_t k_base,
+                             const unsigned long long* block_off, uint64_t* out, hipStream_t st);
+
+}  // namespace voss

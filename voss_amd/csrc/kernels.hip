@@ -1,0 +1,986 @@
+// kernels.hip — gfx950 kernels of the wheel-30 segmented sieve.
+//
+// One workgroup sieves one segment (a run of bitmap bytes, one byte per 30
+// integers) held entirely in LDS, then walks it in order to produce counts,
+// Chebyshev tallies, the consecutive-prime gap histogram, the segment's first
+// and last prime and its largest gap.  A second pass (stitch_kernel) joins
+// segments: it adds each boundary gap (first prime of a segment minus the last
+// prime before it) and flags the few segments that hold maximal-gap records,
+// which rescan_kernel then lists in order.
+//
+// Reference behaviour these kernels reproduce (library/src/primes/):
+//   sieve.cu:10-37        crossing-off (here: LDS ds_and_b32, offsets per lane)
+//   extract.cu:10-39      per-segment prime count (here: popcounts)
+//   gaps.cu:31-75         gap histogram + large gaps (here: in-order LDS walk)
+//   chebyshev.cu:11-25    mod-4 tallies (here: per-bit-plane masked popcounts)
+//   context.cu:462-501    boundary stitching (here: stitch_kernel)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace voss {
+namespace {
+
+constexpr uint32_t kW30Lo = 0x0D0B0701u;  // W30[0..3] = 1, 7, 11, 13
+constexpr uint32_t kW30Hi = 0x1D171311u;  // W30[4..7] = 17, 19, 23, 29
+
+// W30[o] for o in 0..7 (one v_perm_b32: selector bytes 1..3 = 0x0C give 0).
+__device__ __forceinline__ uint32_t w30(uint32_t o) {
+  return __builtin_amdgcn_perm(kW30Hi, kW30Lo, o | 0x0C0C0C00u);
+}
+
+__device__ __forceinline__ uint32_t w30_const(int o) {
+  return o == 0 ? 1 : o == 1 ? 7 : o == 2 ? 11 : o == 3 ? 13 : o == 4 ? 17 : o == 5 ? 19 : o == 6 ? 23 : 29;
+}
+
+// LDS atomic AND without return (ds_and_b32).
+__device__ __forceinline__ void lds_and(uint32_t* p, uint32_t m) {
+  __hip_atomic_fetch_and(p, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// k mod p for k < 2^52, p < 2^24, from a Newton-refined FP64 reciprocal.
+__device__ __forceinline__ uint32_t mod_u64(uint64_t k, uint32_t p) {
+  double pd = (double)p;
+  double inv = __builtin_amdgcn_rcp(pd);
+  double e = __builtin_fma(-pd, inv, 1.0);
+  inv = __builtin_fma(inv, e, inv);
+  uint64_t q = (uint64_t)((double)k * inv);
+  int64_t r = (int64_t)(k - q * (uint64_t)p);
+  if (r < 0) r += p;
+  if (r >= (int64_t)p) r -= p;
+  return (uint32_t)r;
+}
+
+__device__ __forceinline__ uint64_t isqrt_u64(uint64_t n) {
+  uint64_t r = (uint64_t)__builtin_sqrt((double)n);
+  while (r * r > n) --r;
+  while ((r + 1) * (r + 1) <= n) ++r;
+  return r;
+}
+
+// Number of wheel residues W30[o] that are < r (r in 0..30).
+__device__ __forceinline__ uint32_t residues_below(uint32_t r) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) c += (w30_const(o) < r) ? 1u : 0u;
+  return c;
+}
+
+// Bit index (within a segment) of the first candidate >= v, clamped to [0, 8*nbytes].
+__device__ __forceinline__ uint32_t bit_at_or_above(uint64_t v, uint64_t k0, uint32_t nbytes) {
+  if (v <= 30 * k0) return 0;
+  uint64_t k = v / 30;
+  if (k >= k0 + nbytes) return 8 * nbytes;
+  return 8 * (uint32_t)(k - k0) + residues_below((uint32_t)(v % 30));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------- LDS layout
+// u32 word offsets into dynamic LDS.
+constexpr uint32_t kLdsHist = 0;                                  // kHistBins*kHistCopies
+constexpr uint32_t kLdsNib = kLdsHist + kHistBins * kHistCopies;  // 256 nibble lists
+constexpr uint32_t kLdsTab = kLdsNib + 256;                       // 3 x 64 bytes = 48 words
+constexpr uint32_t kLdsMisc = kLdsTab + 48;                       // 256 words
+constexpr uint32_t kLdsSeg = kLdsMisc + 256;                      // segment bytes
+}  // namespace
+
+uint32_t lds_bytes_for(uint32_t seg_bytes) { return (kLdsSeg * 4) + seg_bytes + 16; }
+
+namespace {
+
+// misc word layout
+constexpr uint32_t kMiscItem = 0;       // current work item
+constexpr uint32_t kMiscWave = 8;       // 8 words per wave below
+// per wave: [0] first, [1] last, [2] count, [3] c41, [4] c31, [5] maxgap,
+//           [6] lower count, [7] lower c41 ... (lower c31 in [kMiscWave + 8*NW])
+
+struct SegGeom {
+  uint64_t k0;
+  uint32_t nbytes;
+  uint32_t vlo, vhi;      // valid bit range [vlo, vhi)
+  uint64_t lo_val, hi_val;
+  uint32_t query;
+};
+
+// Initialise tables in LDS (once per workgroup).
+__device__ void init_tables(uint32_t* lds) {
+  const uint32_t tid = threadIdx.x;
+  uint8_t* tab = reinterpret_cast<uint8_t*>(lds + kLdsTab);
+  if (tid < 256) {
+    // internal gaps of a byte, in order, as 4-bit (gap/2) nibbles
+    uint32_t x = tid, nib = 0, sh = 0;
+    int prev = -1;
+    for (int o = 0; o < 8; ++o) {
+      if ((x >> o) & 1) {
+        if (prev >= 0) { nib |= ((w30_const(o) - w30_const(prev)) >> 1) << sh; sh += 4; }
+        prev = o;
+      }
+    }
+    lds[kLdsNib + tid] = nib;
+  }
+  if (tid < 64) {
+    // r = index of p mod 30 in W30, c = class (multiplier m = W30[c] mod 30)
+    uint32_t r = tid >> 3, c = tid & 7;
+    uint32_t prod = w30_const(r) * w30_const(c);
+    uint32_t res = prod % 30, bit = 0;
+    for (int o = 0; o < 8; ++o) if (w30_const(o) == res) bit = o;
+    tab[tid] = (uint8_t)(prod / 30);                                     // FRAC
+    tab[64 + tid] = (uint8_t)bit;                                        // BITS
+    tab[128 + tid] = (uint8_t)((w30_const(c) + 30 - w30_const(r)) % 30); // DM
+  }
+}
+
+// Copy the presieved pattern (primes 7..19 removed) into LDS for the segment,
+// zero-filling up to the next 1 KiB so later passes read whole 16-byte chunks.
+template <int BS>
+__device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
+  uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
+  const uint32_t off = (uint32_t)(g.k0 % kPresievePeriod);
+  const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
+  for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
+    uint32_t src = off + 16 * t;
+    const uint32_t* p = A.presieve + (src >> 2);
+    uint32_t sh = src & 3;
+    uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
+    uint4 v;
+    v.x = __builtin_amdgcn_alignbyte(a1, a0, sh);
+    v.y = __builtin_amdgcn_alignbyte(a2, a1, sh);
+    v.z = __builtin_amdgcn_alignbyte(a3, a2, sh);
+    v.w = __builtin_amdgcn_alignbyte(a4, a3, sh);
+    uint32_t b0 = 16 * t;
+    if (b0 + 16 > g.nbytes) {
+      uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        uint32_t wb = b0 + 4 * i;
+        uint32_t keep = (wb >= g.nbytes) ? 0u : (g.nbytes - wb >= 4 ? 4u : g.nbytes - wb);
+        w[i] &= keep >= 4 ? 0xFFFFFFFFu : ((1u << (8 * keep)) - 1u);
+      }
+      v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
+    }
+    if (g.k0 == 0 && t == 0) v.x = (v.x & ~0xFFu) | 0xFEu;  // 1 is not prime; 7..29 are
+    *reinterpret_cast<uint4*>(seg8 + b0) = v;
+  }
+}
+
+// Cross off every sieving prime >= 23 with p*p <= the segment's top value.
+template <int BS>
+__device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
+  constexpr int NW = BS / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
+  const uint8_t* tab = reinterpret_cast<const uint8_t*>(lds + kLdsTab);
+  const uint64_t k0 = g.k0;
+  const uint32_t nbytes = g.nbytes;
+
+  uint64_t top = 30 * (k0 + nbytes) - 1;
+  if (g.hi_val < top) top = g.hi_val;
+  uint64_t sq = isqrt_u64(top);
+  uint64_t ti = sq / kPiTabStep + 1;
+  if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
+  const uint32_t np = A.pi_tab[ti];
+  const uint32_t n1 = min(np, A.tier1_end);  // primes [0, n1): tier 1
+
+  // ---- tier 1: one prime per wave; lanes = 8 classes x 8 sub-ranges
+  {
+    const uint32_t c = lane & 7, sr = lane >> 3;
+    const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;  // sub-range length, word aligned
+    const uint32_t lo = min(sr * sub_len, nbytes);
+    const uint32_t hi = min(lo + sub_len, nbytes);
+    for (uint32_t i = wave; i < n1; i += NW) {
+      const uint32_t e = A.primes[i];
+      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+      const uint32_t rk = mod_u64(k0, p);
+      const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
+      int32_t off = (int32_t)cc - (int32_t)rk;
+      if (off < 0) off += p;
+      uint64_t offl = (uint32_t)off;
+      if (30 * k0 < (uint64_t)p * p + 30ull * p) {  // the p*p rule (rare: low segments)
+        uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+        uint64_t fb = (uint64_t)p * m / 30;
+        if (k0 + offl < fb) offl = fb - k0;
+      }
+      if (offl >= hi) continue;
+      uint32_t j = (uint32_t)offl;
+      if (j < lo) j += ((lo - j + p - 1) / p) * p;
+      const uint32_t bit = tab[64 + r * 8 + c];
+      // four crossings per turn: byte-in-word pattern repeats every 4 steps
+      const uint32_t m0 = ~(1u << (((j) & 3) * 8 + bit));
+      const uint32_t m1 = ~(1u << (((j + p) & 3) * 8 + bit));
+      const uint32_t m2 = ~(1u << (((j + 2 * p) & 3) * 8 + bit));
+      const uint32_t m3 = ~(1u << (((j + 3 * p) & 3) * 8 + bit));
+      while (j + 3 * p < hi) {
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 3 * p) & ~3u)), m3);
+        j += 4 * p;
+      }
+      if (j < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
+      if (j + p < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
+      if (j + 2 * p < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
+    }
+  }
+
+  // ---- tier 2: 8 primes per wave turn; lane = (prime, class); walk the segment
+  {
+    const uint32_t c = lane & 7;
+    const uint32_t t2end = min(np, A.tier2_end);
+    for (uint32_t base = n1 + 8 * wave; base < t2end; base += 8 * NW) {
+      const uint32_t i = base + (lane >> 3);
+      if (i >= t2end) continue;
+      const uint32_t e = A.primes[i];
+      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+      const uint32_t rk = mod_u64(k0, p);
+      const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
+      int32_t off = (int32_t)cc - (int32_t)rk;
+      if (off < 0) off += p;
+      uint64_t offl = (uint32_t)off;
+      if (30 * k0 < (uint64_t)p * p + 30ull * p) {
+        uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+        uint64_t fb = (uint64_t)p * m / 30;
+        if (k0 + offl < fb) offl = fb - k0;
+      }
+      if (offl >= nbytes) continue;
+      uint32_t j = (uint32_t)offl;
+      const uint32_t bit = tab[64 + r * 8 + c];
+      const uint32_t m0 = ~(1u << (((j) & 3) * 8 + bit));
+      const uint32_t m1 = ~(1u << (((j + p) & 3) * 8 + bit));
+      const uint32_t m2 = ~(1u << (((j + 2 * p) & 3) * 8 + bit));
+      const uint32_t m3 = ~(1u << (((j + 3 * p) & 3) * 8 + bit));
+      while (j + 3 * p < nbytes) {
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
+        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 3 * p) & ~3u)), m3);
+        j += 4 * p;
+      }
+      if (j < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
+      if (j + p < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
+      if (j + 2 * p < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
+    }
+  }
+
+  // ---- tier 3: primes >= segment length; lane = prime, at most one crossing per class
+  {
+    const uint32_t t3beg = min(np, A.tier2_end);
+    for (uint32_t base = t3beg + 64 * wave; base < np; base += 64 * NW) {
+      const uint32_t i = base + lane;
+      if (i >= np) continue;
+      const uint32_t e = A.primes[i];
+      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+      const uint32_t rk = mod_u64(k0, p);
+      const uint32_t q30 = p / 30;
+      const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
+      const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+      const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
+        int32_t off = (int32_t)cc - (int32_t)rk;
+        if (off < 0) off += p;
+        uint64_t offl = (uint32_t)off;
+        if (low) {
+          uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+          uint64_t fb = (uint64_t)p * m / 30;
+          if (k0 + offl < fb) offl = fb - k0;
+        }
+        if (offl < nbytes) {
+          uint32_t j = (uint32_t)offl;
+          uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
+          lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), ~(1u << ((j & 3) * 8 + bit)));
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void hist_add_lds(uint32_t* lds, uint32_t h, uint32_t copy) {
+  lds_add(lds + kLdsHist + h * kHistCopies + copy, 1u);
+}
+
+// Mask a 16-byte chunk (bit offset cb) to the valid bit range [vlo, vhi).
+__device__ __forceinline__ void mask_chunk(uint4& q, uint32_t cb, uint32_t vlo, uint32_t vhi) {
+  uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t b = cb + 32 * i;
+    uint32_t m = 0xFFFFFFFFu;
+    if (vlo > b) m = (vlo - b >= 32) ? 0u : (m << (vlo - b));
+    if (vhi < b + 32) {
+      uint32_t keep = (vhi <= b) ? 0u : vhi - b;
+      m &= keep == 0 ? 0u : (0xFFFFFFFFu >> (32 - keep));
+    }
+    w[i] &= m;
+  }
+  q.x = w[0]; q.y = w[1]; q.z = w[2]; q.w = w[3];
+}
+
+// Bits of a 16-byte chunk (bit offset cb) below bit index split.
+__device__ __forceinline__ uint4 below_split(uint4 q, uint32_t cb, uint32_t split) {
+  uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    uint32_t b = cb + 32 * i;
+    uint32_t keep = split <= b ? 0u : (split - b >= 32 ? 32u : split - b);
+    w[i] &= keep == 0 ? 0u : (0xFFFFFFFFu >> (32 - keep));
+  }
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+__device__ __forceinline__ uint32_t popc4(uint4 q) {
+  return __builtin_popcount(q.x) + __builtin_popcount(q.y) + __builtin_popcount(q.z) + __builtin_popcount(q.w);
+}
+__device__ __forceinline__ uint32_t popc4m(uint4 q, uint32_t m) {
+  return __builtin_popcount(q.x & m) + __builtin_popcount(q.y & m) + __builtin_popcount(q.z & m) +
+         __builtin_popcount(q.w & m);
+}
+
+// Segment-relative value of the highest set bit of a chunk at byte offset boff (0 if empty).
+__device__ __forceinline__ uint32_t chunk_last(uint4 q, uint32_t boff) {
+  uint32_t w = q.w, wi = 3;
+  if (!w) { w = q.z; wi = 2; }
+  if (!w) { w = q.y; wi = 1; }
+  if (!w) { w = q.x; wi = 0; }
+  if (!w) return 0;
+  uint32_t bit = 31 - __builtin_clz(w);
+  return 30 * (boff + 4 * wi + (bit >> 3)) + w30(bit & 7);
+}
+
+// Exclusive "last prime before this lane" within one 64-lane step.
+__device__ __forceinline__ uint32_t prev_in_step(uint32_t lastv, uint32_t lane, uint32_t& step_last) {
+  uint32_t incl = lastv;
+  const bool dense = __ballot(lastv == 0) == 0ull;
+  if (dense) {
+    // every lane holds a prime: the predecessor is simply lane-1's last
+    step_last = __shfl((int)lastv, 63, 64);
+    uint32_t up = (uint32_t)__shfl_up((int)lastv, 1, 64);
+    return lane ? up : 0u;
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+    if (lane >= (uint32_t)d) incl = max(incl, t);
+  }
+  step_last = __shfl((int)incl, 63, 64);
+  uint32_t up = (uint32_t)__shfl_up((int)incl, 1, 64);
+  return lane ? up : 0u;
+}
+
+template <int MODE>
+__device__ __forceinline__ void note_gap(const SieveArgs& A, uint32_t* lds, uint32_t gap, uint32_t vrel,
+                                          const SegGeom& g, uint32_t copy, uint32_t& maxg) {
+  uint32_t h = gap >> 1;
+  if (h < kHistBins) {
+    hist_add_lds(lds, h, copy);
+  } else if (MODE == kModeStats) {
+    atomicAdd(A.hist + (h < kGapBins ? h : kGapBins - 1), 1ull);
+  } else {
+    atomicAdd(A.qhist + (size_t)g.query * kQueryBins, 1u);
+  }
+  maxg = max(maxg, gap);
+  if (MODE == kModeStats && gap >= A.large_threshold) {
+    unsigned int slot = atomicAdd(A.n_large, 1u);
+    if (slot < A.large_cap) {
+      A.large[slot].position = 30 * g.k0 + vrel;
+      A.large[slot].gap = gap;
+      A.large[slot].seg = 0;
+    }
+  }
+}
+
+// In-order walk of the sieved segment: counts, tallies, gaps; per-wave
+// summaries into LDS misc.  Returns nothing; thread 0 merges afterwards.
+template <int MODE, int BS>
+__device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
+  constexpr int NW = BS / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+  const uint32_t copy = lane >> 4;
+  const uint32_t wave_bytes = A.seg_bytes / NW;
+  const uint32_t M41 = (g.k0 & 1) ? 0x99669966u : 0x66996699u;
+  const uint32_t M31 = 0x2B2B2B2Bu;
+  const bool partial = g.vlo > 0 || g.vhi < 8 * g.nbytes;
+
+  uint32_t carry = 0, wfirst = 0;
+  uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
+  const uint32_t wbeg = wave * wave_bytes;
+  const uint32_t wend = min(wbeg + wave_bytes, g.nbytes);
+  for (uint32_t base = wbeg; base < wend; base += 1024) {
+    const uint32_t boff = base + 16 * lane;
+    uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
+    if (partial) {
+      const uint32_t cb = 8 * boff;
+      if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+    }
+    cnt += popc4(q);
+    if (MODE == kModeStats) {
+      c41 += popc4m(q, M41);
+      c31 += popc4m(q, M31);
+      if (split != 0xFFFFFFFFu) {
+        uint4 lo = below_split(q, 8 * boff, split);
+        lcnt += popc4(lo);
+        l41 += popc4m(lo, M41);
+        l31 += popc4m(lo, M31);
+      }
+    }
+    if (MODE == kModeCount) continue;
+    const uint32_t lastv = chunk_last(q, boff);
+    uint32_t step_last;
+    uint32_t prev = prev_in_step(lastv, lane, step_last);
+    prev = max(prev, carry);
+    carry = max(carry, step_last);
+
+    const uint32_t vb = 30 * boff;
+    const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+      if (x) {
+        const uint32_t fo = __builtin_ctz(x);
+        const uint32_t vf = vb + 30 * j + w30(fo);
+        if (prev) {
+          note_gap<MODE>(A, lds, vf - prev, vf, g, copy, maxg);
+        } else {
+          wfirst = vf;
+        }
+        uint32_t nib = lds[kLdsNib + x];
+        while (nib) {
+          const uint32_t h = nib & 15;
+          hist_add_lds(lds, h, copy);
+          maxg = max(maxg, 2 * h);
+          nib >>= 4;
+        }
+        prev = vb + 30 * j + w30(31 - __builtin_clz(x));
+      }
+    }
+  }
+  // per-wave reductions
+  cnt = wave_sum(cnt);
+  if (MODE == kModeStats) {
+    c41 = wave_sum(c41);
+    c31 = wave_sum(c31);
+    lcnt = wave_sum(lcnt);
+    l41 = wave_sum(l41);
+    l31 = wave_sum(l31);
+  }
+  if (MODE != kModeCount) {
+    maxg = wave_max(maxg);
+    wfirst = wave_max(wfirst);
+  }
+  if (lane == 0) {
+    uint32_t* m = lds + kLdsMisc + kMiscWave + 8 * wave;
+    m[0] = wfirst; m[1] = carry; m[2] = cnt; m[3] = c41; m[4] = c31; m[5] = maxg;
+    m[6] = lcnt; m[7] = l41;
+    lds[kLdsMisc + kMiscWave + 8 * NW + wave] = l31;
+  }
+}
+
+template <int BS>
+__device__ void flush_hist_stats(const SieveArgs& A, uint32_t* lds) {
+  for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kHistCopies; ++c) {
+      s += lds[kLdsHist + h * kHistCopies + c];
+      lds[kLdsHist + h * kHistCopies + c] = 0;
+    }
+    if (s) atomicAdd(A.hist + h, (unsigned long long)s);
+  }
+}
+
+template <int BS>
+__device__ void flush_hist_query(const SieveArgs& A, uint32_t* lds, uint32_t qi) {
+  unsigned int* qh = A.qhist + (size_t)qi * kQueryBins;
+  for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kHistCopies; ++c) {
+      s += lds[kLdsHist + h * kHistCopies + c];
+      lds[kLdsHist + h * kHistCopies + c] = 0;
+    }
+    if (s) atomicAdd(qh + ((h >= 1 && h <= 256) ? h : 0u), s);
+  }
+}
+
+__device__ __forceinline__ SegGeom geometry(const SieveArgs& A, uint32_t s, int mode) {
+  SegGeom g;
+  if (mode == kModeBatch) {
+    const uint32_t qi = A.seg_query[s];
+    const QueryDesc Q = A.queries[qi];
+    g.query = qi;
+    g.k0 = Q.k_lo + (uint64_t)(s - Q.seg_first) * A.seg_bytes;
+    uint64_t rem = Q.k_hi - g.k0;
+    g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
+    g.lo_val = Q.a;
+    g.hi_val = Q.b;
+  } else {
+    g.query = 0;
+    g.k0 = A.k_base + (uint64_t)(A.seg_base + s) * A.seg_bytes;
+    uint64_t rem = A.k_end - g.k0;
+    g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
+    g.lo_val = A.lo_val;
+    g.hi_val = A.hi_val;
+  }
+  g.vlo = bit_at_or_above(g.lo_val, g.k0, g.nbytes);
+  g.vhi = bit_at_or_above(g.hi_val + 1, g.k0, g.nbytes);
+  return g;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- main kernel
+template <int MODE, int BS>
+__global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
+  extern __shared__ uint32_t lds[];
+  constexpr int NW = BS / 64;
+  init_tables(lds);
+  for (uint32_t i = threadIdx.x; i < kHistBins * kHistCopies; i += BS) lds[kLdsHist + i] = 0;
+  uint32_t cur_query = 0xFFFFFFFFu;
+  uint32_t since_flush = 0;
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0) lds[kLdsMisc + kMiscItem] = atomicAdd(A.work, 1u);
+    __syncthreads();
+    const uint32_t s = lds[kLdsMisc + kMiscItem];
+    if (s >= A.nseg) break;
+    const SegGeom g = geometry(A, s, MODE);
+    if (MODE == kModeBatch && g.query != cur_query) {
+      if (cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
+      cur_query = g.query;
+    }
+    init_segment<BS>(A, lds, g);
+    __syncthreads();
+    sieve_segment<BS>(A, lds, g);
+    __syncthreads();
+    if (MODE == kModeBitmap) {
+      const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+      uint8_t* dst = A.bitmap + (g.k0 - A.k_base);
+      const uint32_t n16 = (g.nbytes + 15) >> 4;
+      for (uint32_t t = threadIdx.x; t < n16; t += BS) {
+        uint4 q = *reinterpret_cast<const uint4*>(seg8 + 16 * t);
+        const uint32_t cb = 128 * t;
+        if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+        *reinterpret_cast<uint4*>(dst + 16 * t) = q;
+      }
+      continue;
+    }
+
+    // checkpoint split (stats mode): at most one bucket boundary per segment
+    uint32_t split = 0xFFFFFFFFu;
+    uint32_t b0 = 0, b1 = 0;
+    if (MODE == kModeStats && A.cp_interval) {
+      const uint64_t vfirst = 30 * g.k0 + 1;
+      const uint64_t vlast = 30 * (g.k0 + g.nbytes) - 1;
+      b0 = (uint32_t)((vfirst - 1) / A.cp_interval);
+      b1 = (uint32_t)((vlast - 1) / A.cp_interval);
+      if (b1 != b0) split = bit_at_or_above((uint64_t)(b0 + 1) * A.cp_interval + 1, g.k0, g.nbytes);
+    }
+    walk_segment<MODE, BS>(A, lds, g, split);
+    __syncthreads();
+
+    if (threadIdx.x == 0) {
+      const uint32_t* m = lds + kLdsMisc + kMiscWave;
+      uint32_t sfirst = 0, slast = 0, smax = 0, scnt = 0, s41 = 0, s31 = 0, lc = 0, l41 = 0, l31 = 0;
+      for (int w = 0; w < NW; ++w) {
+        const uint32_t* mw = m + 8 * w;
+        scnt += mw[2];
+        if (MODE == kModeStats) { s41 += mw[3]; s31 += mw[4]; lc += mw[6]; l41 += mw[7]; l31 += m[8 * NW + w]; }
+        if (MODE != kModeCount && mw[2]) {
+          smax = max(smax, mw[5]);
+          if (slast) {
+            uint32_t gap = mw[0] - slast;
+            uint32_t dummy = 0;
+            note_gap<MODE>(A, lds, gap, mw[0], g, 0, dummy);
+            smax = max(smax, gap);
+          } else {
+            sfirst = mw[0];
+          }
+          slast = mw[1];
+        }
+      }
+      if (A.summaries) {
+        SegSummary ss;
+        ss.first = sfirst; ss.last = slast; ss.maxgap = smax; ss.count = scnt;
+        A.summaries[(MODE == kModeBatch) ? s : A.seg_base + s] = ss;
+      }
+      if (MODE == kModeBatch) {
+        atomicAdd(A.qcount + g.query, (unsigned long long)scnt);
+      } else {
+        atomicAdd(A.totals + 0, (unsigned long long)scnt);
+      }
+      if (MODE == kModeStats) {
+        atomicAdd(A.totals + 1, (unsigned long long)s41);
+        atomicAdd(A.totals + 2, (unsigned long long)s31);
+        if (A.cp_interval && b0 < A.nbuckets) {
+          unsigned long long* bk = A.buckets;
+          if (split == 0xFFFFFFFFu) {
+            atomicAdd(bk + 3 * b0 + 0, (unsigned long long)scnt);
+            atomicAdd(bk + 3 * b0 + 1, (unsigned long long)s41);
+            atomicAdd(bk + 3 * b0 + 2, (unsigned long long)s31);
+          } else {
+            atomicAdd(bk + 3 * b0 + 0, (unsigned long long)lc);
+            atomicAdd(bk + 3 * b0 + 1, (unsigned long long)l41);
+            atomicAdd(bk + 3 * b0 + 2, (unsigned long long)l31);
+            if (b1 < A.nbuckets) {
+              atomicAdd(bk + 3 * b1 + 0, (unsigned long long)(scnt - lc));
+              atomicAdd(bk + 3 * b1 + 1, (unsigned long long)(s41 - l41));
+              atomicAdd(bk + 3 * b1 + 2, (unsigned long long)(s31 - l31));
+            }
+          }
+        }
+      }
+    }
+    if (MODE == kModeStats && ++since_flush == 1024) {
+      __syncthreads();
+      flush_hist_stats<BS>(A, lds);
+      since_flush = 0;
+    }
+  }
+  if (MODE == kModeStats) flush_hist_stats<BS>(A, lds);
+  if (MODE == kModeBatch && cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
+}
+
+// ---------------------------------------------------------------- stitch (sweep)
+// One workgroup joins the per-segment summaries in order: boundary gaps go
+// to the histogram and large-gap list; boundary records are emitted
+// directly; segments whose internal largest gap beats every earlier gap are
+// listed for rescan_kernel together with the running maximum before them.
+__global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
+  __shared__ unsigned long long s_last[1024];
+  __shared__ uint32_t s_max[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (A.nseg + 1023) / 1024;
+  const uint32_t lo = min(t * per, A.nseg), hi = min(lo + per, A.nseg);
+  auto absval = [&](uint32_t s, uint32_t rel) -> uint64_t {
+    return 30ull * (A.k_base + (uint64_t)s * A.seg_bytes) + rel;
+  };
+  uint64_t last = 0;
+  uint32_t emax = 0;
+  for (uint32_t s = lo; s < hi; ++s) {
+    const SegSummary S = A.summaries[s];
+    emax = max(emax, S.maxgap);
+    if (!S.count) continue;
+    const uint64_t F = absval(s, S.first);
+    if (last) emax = max(emax, (uint32_t)(F - last));
+    last = absval(s, S.last);
+  }
+  s_last[t] = last;
+  __syncthreads();
+  if (t == 0) {  // exclusive running "last prime before" across threads
+    unsigned long long run = A.seed_last;
+    for (int i = 0; i < 1024; ++i) {
+      unsigned long long v = s_last[i];
+      s_last[i] = run;
+      if (v) run = v;
+    }
+  }
+  __syncthreads();
+  const uint64_t prev0 = s_last[t];
+  // boundary gap of this thread's first non-empty segment
+  for (uint32_t s = lo; s < hi; ++s) {
+    const SegSummary S = A.summaries[s];
+    if (!S.count) continue;
+    emax = max(emax, (uint32_t)(absval(s, S.first) - prev0));
+    break;
+  }
+  s_max[t] = emax;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = A.seed_max;
+    for (int i = 0; i < 1024; ++i) {
+      uint32_t v = s_max[i];
+      s_max[i] = run;
+      run = max(run, v);
+    }
+  }
+  __syncthreads();
+  uint32_t run = s_max[t];
+  uint64_t prev = prev0;
+  for (uint32_t s = lo; s < hi; ++s) {
+    const SegSummary S = A.summaries[s];
+    if (!S.count) continue;
+    const uint64_t F = absval(s, S.first);
+    const uint32_t B = (uint32_t)(F - prev);
+    const uint32_t h = B >> 1;
+    atomicAdd(A.hist + (h < kGapBins ? h : kGapBins - 1), 1ull);
+    if (B >= A.large_threshold) {
+      unsigned int slot = atomicAdd(A.n_large, 1u);
+      if (slot < A.large_cap) { A.large[slot].position = F; A.large[slot].gap = B; A.large[slot].seg = s; }
+    }
+    if (B > run) {
+      unsigned int slot = atomicAdd(A.n_rec, 1u);
+      if (slot < A.rec_cap) { A.rec[slot].position = F; A.rec[slot].gap = B; A.rec[slot].seg = s; }
+      run = B;
+    }
+    if (S.maxgap > run) {
+      unsigned int slot = atomicAdd(A.n_rescan, 1u);
+      if (slot < A.rescan_cap) { A.rescan[slot].seg = s; A.rescan[slot].threshold = run; }
+      run = S.maxgap;
+    }
+    prev = absval(s, S.last);
+  }
+  if (t == 1023) {
+    A.final_out[0] = prev;
+    A.final_out[1] = run;
+  }
+}
+
+// ---------------------------------------------------------------- records rescan
+// One workgroup per flagged segment: re-sieve it, then repeatedly find the
+// first gap above the running maximum (which then becomes the new maximum).
+template <int BS>
+__global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
+  extern __shared__ uint32_t lds[];
+  constexpr int NW = BS / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ unsigned long long s_best;
+  init_tables(lds);
+  __syncthreads();
+  const RescanItem item = R.items[blockIdx.x];
+  const uint32_t s = item.seg;
+  SegGeom g;
+  g.query = 0;
+  g.k0 = A.k_base + (uint64_t)s * A.seg_bytes;
+  uint64_t rem = A.k_end - g.k0;
+  g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
+  g.lo_val = A.lo_val;
+  g.hi_val = A.hi_val;
+  g.vlo = bit_at_or_above(g.lo_val, g.k0, g.nbytes);
+  g.vhi = bit_at_or_above(g.hi_val + 1, g.k0, g.nbytes);
+  init_segment<BS>(A, lds, g);
+  __syncthreads();
+  sieve_segment<BS>(A, lds, g);
+  __syncthreads();
+  const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+  const uint32_t wave_bytes = A.seg_bytes / NW;
+  uint32_t thr = item.threshold;
+  uint32_t nrec = 0;
+  for (int iter = 0; iter < 64; ++iter) {
+    if (threadIdx.x == 0) s_best = ~0ull;
+    __syncthreads();
+    uint32_t carry = 0, wfirst = 0;
+    unsigned long long best = ~0ull;
+    const uint32_t wbeg = wave * wave_bytes, wend = min(wbeg + wave_bytes, g.nbytes);
+    for (uint32_t base = wbeg; base < wend; base += 1024) {
+      const uint32_t boff = base + 16 * lane;
+      uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
+      const uint32_t cb = 8 * boff;
+      if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+      uint32_t step_last;
+      uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
+      carry = max(carry, step_last);
+      const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+      for (int wi = 0; wi < 4; ++wi) {
+        uint32_t w = ws[wi];
+        while (w) {
+          const uint32_t b = __builtin_ctz(w);
+          w &= w - 1;
+          const uint32_t v = 30 * (boff + 4 * wi + (b >> 3)) + w30(b & 7);
+          if (prev) {
+            if (v - prev > thr && best == ~0ull) best = ((unsigned long long)v << 20) | (v - prev);
+          } else {
+            wfirst = v;
+          }
+          prev = v;
+        }
+      }
+    }
+    wfirst = wave_max(wfirst);
+    if (best != ~0ull) atomicMin(&s_best, best);
+    if (lane == 0) {
+      lds[kLdsMisc + kMiscWave + 2 * wave] = wfirst;
+      lds[kLdsMisc + kMiscWave + 2 * wave + 1] = carry;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t last = 0;
+      unsigned long long b = s_best;
+      for (int w = 0; w < NW; ++w) {
+        uint32_t f = lds[kLdsMisc + kMiscWave + 2 * w], l = lds[kLdsMisc + kMiscWave + 2 * w + 1];
+        if (!l) continue;
+        if (last && f - last > thr) {
+          unsigned long long c = ((unsigned long long)f << 20) | (f - last);
+          if (c < b) b = c;
+        }
+        last = l;
+      }
+      s_best = b;
+    }
+    __syncthreads();
+    const unsigned long long b = s_best;
+    __syncthreads();
+    if (b == ~0ull) break;
+    const uint32_t v = (uint32_t)(b >> 20), gap = (uint32_t)(b & 0xFFFFF);
+    if (threadIdx.x == 0) {
+      unsigned int slot = atomicAdd(R.n_out, 1u);
+      if (slot < R.out_cap) {
+        R.out[slot].position = 30 * g.k0 + v;
+        R.out[slot].gap = gap;
+        R.out[slot].seg = s;
+      }
+    }
+    thr = gap;
+    ++nrec;
+  }
+}
+
+// ---------------------------------------------------------------- batch stitch
+// One thread per query: join its segments in order (boundary gaps into the
+// query histogram) and report first and last prime.
+__global__ void batch_stitch_kernel(BatchStitchArgs A) {
+  const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= A.nq) return;
+  const QueryDesc Q = A.queries[qi];
+  uint64_t first = 0, last = 0;
+  unsigned int* qh = A.qhist + (size_t)qi * kQueryBins;
+  for (uint32_t i = 0; i < Q.nseg; ++i) {
+    const uint32_t s = Q.seg_first + i;
+    const SegSummary S = A.summaries[s];
+    if (!S.count) continue;
+    const uint64_t base = 30ull * (Q.k_lo + (uint64_t)i * A.seg_bytes);
+    const uint64_t F = base + S.first;
+    if (last) {
+      const uint64_t gap = F - last;
+      const uint64_t h = gap >> 1;
+      atomicAdd(qh + ((h >= 1 && h <= 256 && !(gap & 1)) ? (uint32_t)h : 0u), 1u);
+    } else {
+      first = F;
+    }
+    last = base + S.last;
+  }
+  A.first[qi] = first;
+  A.last[qi] = last;
+}
+
+// ---------------------------------------------------------------- launchers
+template <int MODE, int BS>
+static hipError_t launch_sieve_t(const SieveArgs& A, int grid, hipStream_t st) {
+  hipLaunchKernelGGL((sieve_kernel<MODE, BS>), dim3(grid), dim3(BS), lds_bytes_for(A.seg_bytes), st, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipStream_t st) {
+  if (block == 1024) {
+    if (mode == kModeCount) return launch_sieve_t<kModeCount, 1024>(A, grid, st);
+    if (mode == kModeStats) return launch_sieve_t<kModeStats, 1024>(A, grid, st);
+    if (mode == kModeBitmap) return launch_sieve_t<kModeBitmap, 1024>(A, grid, st);
+    return launch_sieve_t<kModeBatch, 1024>(A, grid, st);
+  }
+  if (mode == kModeCount) return launch_sieve_t<kModeCount, 512>(A, grid, st);
+  if (mode == kModeStats) return launch_sieve_t<kModeStats, 512>(A, grid, st);
+  if (mode == kModeBitmap) return launch_sieve_t<kModeBitmap, 512>(A, grid, st);
+  return launch_sieve_t<kModeBatch, 512>(A, grid, st);
+}
+
+// ---------------------------------------------------------------- bitmap -> sorted list
+// Blocks of kListBlock bitmap bytes; 256 threads x 16 bytes each.
+constexpr uint32_t kListBlock = 4096;
+
+__global__ void __launch_bounds__(256) list_count_kernel(const uint8_t* bm, uint64_t nbytes, uint32_t* counts) {
+  __shared__ uint32_t red[4];
+  const uint64_t off = (uint64_t)blockIdx.x * kListBlock + 16 * threadIdx.x;
+  uint32_t c = 0;
+  if (off < nbytes) c = popc4(*reinterpret_cast<const uint4*>(bm + off));
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void __launch_bounds__(256) list_write_kernel(const uint8_t* bm, uint64_t nbytes, uint64_t k_base,
+                                                         const unsigned long long* block_off, uint64_t* out) {
+  __shared__ uint32_t wsum[4];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t off = (uint64_t)blockIdx.x * kListBlock + 16 * threadIdx.x;
+  uint4 q = make_uint4(0, 0, 0, 0);
+  if (off < nbytes) q = *reinterpret_cast<const uint4*>(bm + off);
+  const uint32_t c = popc4(q);
+  uint32_t incl = c;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+    if (lane >= (uint32_t)d) incl += t;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t w = 0; w < wave; ++w) base += wsum[w];
+  uint64_t pos = block_off[blockIdx.x] + base + incl - c;
+  const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+  for (int wi = 0; wi < 4; ++wi) {
+    uint32_t w = ws[wi];
+    while (w) {
+      const uint32_t b = __builtin_ctz(w);
+      w &= w - 1;
+      out[pos++] = 30ull * (k_base + off + 4 * wi + (b >> 3)) + w30(b & 7);
+    }
+  }
+}
+
+hipError_t launch_list_count(const uint8_t* bm, uint64_t nbytes, uint32_t* counts, hipStream_t st) {
+  const uint64_t nb = (nbytes + kListBlock - 1) / kListBlock;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(list_count_kernel, dim3((uint32_t)nb), dim3(256), 0, st, bm, nbytes, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_list_write(const uint8_t* bm, uint64_t nbytes, uint64_t k_base,
+                             const unsigned long long* block_off, uint64_t* out, hipStream_t st) {
+  const uint64_t nb = (nbytes + kListBlock - 1) / kListBlock;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(list_write_kernel, dim3((uint32_t)nb), dim3(256), 0, st, bm, nbytes, k_base, block_off, out);
+  return hipGetLastError();
+}
+
+hipError_t prepare_kernels(uint32_t lds_bytes) {
+  const void* fns[] = {
+      (const void*)sieve_kernel<kModeCount, 1024>, (const void*)sieve_kernel<kModeStats, 1024>,
+      (const void*)sieve_kernel<kModeBatch, 1024>, (const void*)sieve_kernel<kModeCount, 512>,
+      (const void*)sieve_kernel<kModeStats, 512>,  (const void*)sieve_kernel<kModeBatch, 512>,
+      (const void*)sieve_kernel<kModeBitmap, 1024>, (const void*)sieve_kernel<kModeBitmap, 512>,
+      (const void*)rescan_kernel<1024>,            (const void*)rescan_kernel<512>};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_stitch(const StitchArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(stitch_kernel, dim3(1), dim3(1024), 0, st, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+  if (block == 1024)
+    hipLaunchKernelGGL((rescan_kernel<1024>), dim3(nitems), dim3(1024), lds_bytes_for(A.seg_bytes), st, A, R);
+  else
+    hipLaunchKernelGGL((rescan_kernel<512>), dim3(nitems), dim3(512), lds_bytes_for(A.seg_bytes), st, A, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_stitch(const BatchStitchArgs& A, hipStream_t st) {
+  if (A.nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_stitch_kernel, dim3((A.nq + 255) / 256), dim3(256), 0, st, A);
+  return hipGetLastError();
+}
+
+}  // namespace voss

@@ -1,0 +1,260 @@
+// misc_kernels.hip — gfx950 kernels for the callers around the sieve:
+// Goldbach (existence test and partition counts), Sophie Germain counts over
+// a device-resident wheel bitmap, and a 64-bit Montgomery Miller-Rabin for
+// batched is_prime / next / prev.
+//
+// Reference behaviour (library/src/):
+//   primes/goldbach_upto.cu:25-46   every even n in [4, N] has a partition
+//   primes/goldbach.cpp:21-43       #{p1 <= p2 prime, p1 + p2 = n}
+//   primes/sophie_germain_upto.cu:22-33  p and 2p+1 both prime
+//   core/miller_rabin.cpp:26-51     deterministic MR, bases 2..37
+//   primes/next_prev.cu:22-83       odd scan with MR
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "misc_kernels.h"
+
+namespace voss {
+namespace {
+
+__constant__ uint8_t c_res2bit[30] = {8, 0, 8, 8, 8, 8, 8, 1, 8, 8, 8, 2, 8, 3, 8,
+                                      8, 8, 4, 8, 5, 8, 8, 8, 6, 8, 8, 8, 8, 8, 7};
+constexpr uint32_t kW30Lo = 0x0D0B0701u;
+constexpr uint32_t kW30Hi = 0x1D171311u;
+__device__ __forceinline__ uint32_t w30(uint32_t o) {
+  return __builtin_amdgcn_perm(kW30Hi, kW30Lo, o | 0x0C0C0C00u);
+}
+
+// Primality of v from a wheel bitmap whose byte 0 is k = 0 (covers up to limit).
+__device__ __forceinline__ bool bm_prime(const uint8_t* bm, uint64_t v) {
+  if (v < 7) return v == 2 || v == 3 || v == 5;
+  const uint64_t k = v / 30;
+  const uint32_t b = c_res2bit[(uint32_t)(v - 30 * k)];
+  if (b == 8) return false;
+  return (bm[k] >> b) & 1;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------- Montgomery MR
+struct Mont {
+  uint64_t n, ninv, one, r2;
+};
+
+__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, const Mont& M) {
+  const uint64_t lo = a * b, hi = __umul64hi(a, b);
+  const uint64_t m = lo * M.ninv;
+  const uint64_t mh = __umul64hi(m, M.n);
+  const uint64_t c1 = lo != 0;
+  const uint64_t s1 = hi + mh;
+  const bool o1 = s1 < hi;
+  const uint64_t s2 = s1 + c1;
+  const bool o2 = s2 < s1;
+  uint64_t r = s2;
+  if (o1 || o2 || r >= M.n) r -= M.n;
+  return r;
+}
+
+__device__ Mont mont_setup(uint64_t n) {
+  Mont M;
+  M.n = n;
+  uint64_t inv = n;  // n * n == 1 mod 8 for odd n: 3 correct bits
+#pragma unroll
+  for (int i = 0; i < 5; ++i) inv *= 2 - n * inv;
+  M.ninv = 0 - inv;  // -n^-1 mod 2^64
+  const uint64_t r = (0 - n) % n;  // 2^64 mod n
+  M.one = r;
+  uint64_t r2 = r;  // 2^64 * 2^k mod n, doubled 64 times -> 2^128 mod n
+  for (int i = 0; i < 64; ++i) {
+    const uint64_t t = r2 << 1;
+    const bool of = (r2 >> 63) != 0;
+    r2 = (of || t >= n) ? t - n : t;
+  }
+  M.r2 = r2;
+  return M;
+}
+
+__device__ bool mr_is_prime(uint64_t n) {
+  if (n < 2) return false;
+  if (n == 2 || n == 3) return true;
+  if ((n & 1) == 0) return false;
+  const uint32_t small[11] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (int i = 0; i < 11; ++i) {
+    if (n == small[i]) return true;
+    if (n % small[i] == 0) return false;
+  }
+  uint64_t d = n - 1;
+  int r = 0;
+  while ((d & 1) == 0) { d >>= 1; ++r; }
+  const Mont M = mont_setup(n);
+  const uint64_t minus1 = M.n - M.one;  // -1 in Montgomery form
+  const uint32_t wit[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+  for (int i = 0; i < 12; ++i) {
+    const uint64_t a = wit[i];
+    if (a >= n) continue;
+    uint64_t x = M.one, base = mont_mul(a, M.r2, M);
+    uint64_t e = d;
+    while (e) {
+      if (e & 1) x = mont_mul(x, base, M);
+      base = mont_mul(base, base, M);
+      e >>= 1;
+    }
+    if (x == M.one || x == minus1) continue;
+    bool comp = true;
+    for (int k = 0; k < r - 1; ++k) {
+      x = mont_mul(x, x, M);
+      if (x == minus1) { comp = false; break; }
+    }
+    if (comp) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- kernels
+__global__ void is_prime_kernel(const uint64_t* x, uint64_t n, uint8_t* out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = mr_is_prime(x[i]) ? 1 : 0;
+}
+
+// next: smallest prime > x (x < 2 -> 2); prev: largest prime < x (x <= 2 -> 0).
+__global__ void next_prev_kernel(const uint64_t* x, uint64_t n, uint64_t* out, int dir) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t v = x[i];
+  uint64_t res = 0;
+  if (dir > 0) {
+    if (v < 2) {
+      res = 2;
+    } else {
+      uint64_t c = v + 1;
+      if (c == 2) res = 2;
+      else {
+        if ((c & 1) == 0) ++c;
+        for (uint64_t steps = 0; steps < 100000000ull; steps += 2, c += 2)
+          if (mr_is_prime(c)) { res = c; break; }
+      }
+    }
+  } else {
+    if (v <= 2) res = 0;
+    else if (v == 3) res = 2;
+    else {
+      uint64_t c = v - 1;
+      if (c == 2) res = 2;
+      else {
+        if ((c & 1) == 0) --c;
+        res = 2;
+        for (uint64_t steps = 0; steps < 100000000ull && c >= 3; steps += 2, c -= 2)
+          if (mr_is_prime(c)) { res = c; break; }
+      }
+    }
+  }
+  out[i] = res;
+}
+
+// One thread per even n = 4 + 2*idx: search the ascending small primes for a
+// partner n - p in the bitmap (goldbach_upto.cu:25-46).
+__global__ void goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t N,
+                                      uint64_t* ce, unsigned long long* nce, uint64_t max_ce) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n = 4 + 2 * idx;
+  if (n > N) return;
+  if (n == 4) return;  // 2 + 2
+  const uint64_t half = n / 2;
+  for (uint32_t i = 0; i < nsp; ++i) {
+    const uint64_t p = sp[i];
+    if (p > half) break;
+    if (bm_prime(bm, n - p)) return;
+  }
+  const unsigned long long slot = atomicAdd(nce, 1ull);
+  if (slot < max_ce) ce[slot] = n;
+}
+
+// One workgroup (256 threads) per n: count primes p <= n/2 with n - p prime.
+__global__ void __launch_bounds__(256) goldbach_count_kernel(const uint8_t* bm, const uint64_t* ns,
+                                                             unsigned long long* out) {
+  __shared__ unsigned long long red[4];
+  const uint64_t n = ns[blockIdx.x];
+  const uint64_t half = n / 2;
+  unsigned long long c = 0;
+  if (threadIdx.x == 0) {
+    if (n == 4) c = 1;
+    if (n > 4) {
+      if (3 <= half && bm_prime(bm, n - 3)) ++c;
+      if (5 <= half && bm_prime(bm, n - 5)) ++c;
+    }
+  }
+  if (n > 4) {
+    const uint64_t kmax = half / 30;
+    for (uint64_t k = threadIdx.x; k <= kmax; k += 256) {
+      uint32_t x = bm[k];
+      while (x) {
+        const uint32_t o = __builtin_ctz(x);
+        x &= x - 1;
+        const uint64_t p = 30 * k + w30(o);
+        if (p < 7 || p > half) continue;
+        if (bm_prime(bm, n - p)) ++c;
+      }
+    }
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Sophie Germain: bitmap primes p in [7, N] with 2p + 1 prime.
+__global__ void sophie_kernel(const uint8_t* bm, uint64_t N, unsigned long long* out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (k <= N / 30) {
+    uint32_t x = bm[k];
+    while (x) {
+      const uint32_t o = __builtin_ctz(x);
+      x &= x - 1;
+      const uint64_t p = 30 * k + w30(o);
+      if (p >= 7 && p <= N && bm_prime(bm, 2 * p + 1)) ++c;
+    }
+  }
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, c);
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_is_prime(const uint64_t* x, uint64_t n, uint8_t* out, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(is_prime_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, x, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_next_prev(const uint64_t* x, uint64_t n, uint64_t* out, int dir, hipStream_t st) {
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(next_prev_kernel, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, st, x, n, out, dir);
+  return hipGetLastError();
+}
+hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t N, uint64_t* ce,
+                                 unsigned long long* nce, uint64_t max_ce, hipStream_t st) {
+  if (N < 4) return hipSuccess;
+  const uint64_t cnt = (N - 4) / 2 + 1;
+  hipLaunchKernelGGL(goldbach_check_kernel, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, bm, sp, nsp, N,
+                     ce, nce, max_ce);
+  return hipGetLastError();
+}
+hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, unsigned long long* out,
+                                 hipStream_t st) {
+  if (!k) return hipSuccess;
+  hipLaunchKernelGGL(goldbach_count_kernel, dim3((uint32_t)k), dim3(256), 0, st, bm, ns, out);
+  return hipGetLastError();
+}
+hipError_t launch_sophie(const uint8_t* bm, uint64_t N, unsigned long long* out, hipStream_t st) {
+  const uint64_t nk = N / 30 + 1;
+  hipLaunchKernelGGL(sophie_kernel, dim3((uint32_t)((nk + 255) / 256)), dim3(256), 0, st, bm, N, out);
+  return hipGetLastError();
+}
+
+}  // namespace voss

@@ -1,0 +1,142 @@
+"""Full-statistics sweep and batched range queries (no reference counterpart:
+these expose the one-pass outputs of the MI355X engine; see
+include/voss/voss_amd.h)."""
+
+import ctypes
+
+from . import _capi
+from .primes import _ERRMAP, VossInvalidNError
+
+_L = _capi._lib
+
+
+def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
+               with_hist: bool = True, rec_cap: int = 4096, large_cap: int = 1 << 16) -> dict:
+    """Every statistic of [2, N] from one GPU sweep.
+
+    Returns a dict: prime_count, pi_4_1, pi_4_3, pi_3_1, pi_3_2, last_prime,
+    max_gap, hist (list, index = gap; hist[1] counts 2->3), checkpoints (one
+    tuple (pi, pi_4_1, pi_4_3, pi_3_1, pi_3_2) per x = j * cp_interval),
+    large_gaps ((position, gap) with gap >= large_threshold, largest first),
+    records ((position, gap) maximal gaps ascending), n_large, n_records.
+    """
+    if not isinstance(N, int):
+        raise TypeError("N must be int")
+    if N < 2:
+        raise VossInvalidNError(f"N must be >= 2, got {N}")
+    st = _capi.AmdFullStats()
+    hist = (ctypes.c_uint64 * 100000)() if with_hist else None
+    ncp = N // cp_interval if cp_interval else 0
+    cps = (ctypes.c_uint64 * max(1, 5 * ncp))()
+    lg = (_capi.AmdGap * large_cap)()
+    rec = (_capi.AmdGap * rec_cap)()
+    rc = _L.voss_amd_full_stats(N, cp_interval, large_threshold, ctypes.byref(st), hist,
+                                cps if ncp else None, ncp, lg, large_cap, rec, rec_cap)
+    _capi._check(rc, _ERRMAP)
+    nl, nr = min(st.n_large, large_cap), min(st.n_records, rec_cap)
+    return {
+        "N": N,
+        "prime_count": st.prime_count, "pi_4_1": st.pi_4_1, "pi_4_3": st.pi_4_3,
+        "pi_3_1": st.pi_3_1, "pi_3_2": st.pi_3_2, "last_prime": st.last_prime,
+        "max_gap": st.max_gap, "n_large": st.n_large, "n_records": st.n_records,
+        "hist": list(hist) if with_hist else None,
+        "checkpoints": [tuple(cps[5 * j + f] for f in range(5)) for j in range(ncp)],
+        "large_gaps": [(lg[i].position, lg[i].gap) for i in range(nl)],
+        "records": [(rec[i].position, rec[i].gap) for i in range(nr)],
+    }
+
+
+def _range_dict(r):
+    return {"count": r.count, "first": r.first, "last": r.last, "hist": list(r.hist)}
+
+
+def range_stats(a: int, b: int) -> dict:
+    """Count, first, last and gap histogram (bins 2..512; [0] = other gaps) of
+    the primes in [a, b]: one query, one launch."""
+    out = _capi.AmdRange()
+    _capi._check(_L.voss_amd_range_stats(a, b, ctypes.byref(out)), _ERRMAP)
+    return _range_dict(out)
+
+
+def batch_ranges(a: list, b: list) -> list:
+    """range_stats for many queries in one persistent launch."""
+    n = len(a)
+    if len(b) != n:
+        raise ValueError("a and b must have the same length")
+    out = (_capi.AmdRange * max(1, n))()
+    _capi._check(_L.voss_amd_batch_ranges(_capi.u64_array(a), _capi.u64_array(b), n, out), _ERRMAP)
+    return [_range_dict(out[i]) for i in range(n)]
+
+
+def count_range(a: int, b: int) -> int:
+    out = ctypes.c_uint64(0)
+    _capi._check(_L.voss_amd_count_range(a, b, ctypes.byref(out)), _ERRMAP)
+    return out.value
+
+
+def is_prime_batch(xs: list) -> list:
+    """GPU Miller-Rabin over many u64 values."""
+    n = len(xs)
+    out = (ctypes.c_uint8 * max(1, n))()
+    _capi._check(_L.voss_amd_is_prime_batch(_capi.u64_array(xs), n, out), _ERRMAP)
+    return [bool(out[i]) for i in range(n)]
+
+
+def next_prime_batch(xs: list) -> list:
+    n = len(xs)
+    out = (ctypes.c_uint64 * max(1, n))()
+    _capi._check(_L.voss_amd_next_prime_batch(_capi.u64_array(xs), n, out), _ERRMAP)
+    return list(out[:n])
+
+
+def prev_prime_batch(xs: list) -> list:
+    """Largest prime < x per value (0 for x <= 2)."""
+    n = len(xs)
+    out = (ctypes.c_uint64 * max(1, n))()
+    _capi._check(_L.voss_amd_prev_prime_batch(_capi.u64_array(xs), n, out), _ERRMAP)
+    return list(out[:n])
+
+
+def nth_prime(n: int) -> int:
+    """n-th prime without the 5e7 cap of primes.nth (n <= 1e12)."""
+    out = ctypes.c_uint64(0)
+    _capi._check(_L.voss_amd_nth_prime(n, ctypes.byref(out)), _ERRMAP)
+    return out.value
+
+
+def goldbach_counts(ns: list) -> list:
+    """Goldbach partition counts for many even n in one launch."""
+    k = len(ns)
+    out = (ctypes.c_uint64 * max(1, k))()
+    _capi._check(_L.voss_amd_goldbach_counts(_capi.u64_array(ns), k, out), _ERRMAP)
+    return list(out[:k])
+
+
+def set_config(seg_bytes: int, block: int = 1024, wgs_per_cu: int = 1, tier1_p: int = 1024) -> None:
+    _capi._check(_L.voss_amd_set_config(seg_bytes, block, wgs_per_cu, tier1_p), _ERRMAP)
+
+
+def get_config() -> dict:
+    s, b, w, t = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
+    _capi._check(_L.voss_amd_get_config(ctypes.byref(s), ctypes.byref(b), ctypes.byref(w),
+                                        ctypes.byref(t)), _ERRMAP)
+    return {"seg_bytes": s.value, "block": b.value, "wgs_per_cu": w.value, "tier1_p": t.value}
+
+
+def last_kernel_ms() -> dict:
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    s = ctypes.c_uint64()
+    _capi._check(_L.voss_amd_last_kernel_ms(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                            ctypes.byref(s)), _ERRMAP)
+    return {"sieve_ms": a.value, "stitch_ms": b.value, "total_ms": c.value, "segments": s.value}
+
+
+def device_info() -> dict:
+    n = ctypes.c_int()
+    name = ctypes.create_string_buffer(64)
+    _capi._check(_L.voss_amd_device_info(ctypes.byref(n), name, 64), _ERRMAP)
+    return {"num_cus": n.value, "arch": name.value.decode()}
+
+
+def synchronize() -> None:
+    _capi._check(_L.voss_amd_synchronize(), _ERRMAP)
