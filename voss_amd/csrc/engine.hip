@@ -96,6 +96,7 @@ Engine::Engine() {
   ncu_ = prop.multiProcessorCount;
   name_ = prop.gcnArchName;
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  ws_.resize(kWsCount_);
 
   TuneConfig c;
   c.seg_bytes = env_u32("VOSS_SEG_BYTES", c.seg_bytes);
@@ -198,7 +199,8 @@ int Engine::grid_for(uint32_t nseg) const {
 }
 
 DevBuf& Engine::ws(int slot, size_t bytes) {
-  if (slot >= (int)ws_.size()) ws_.resize(slot + 1);
+  // ws_ is sized once in the constructor: references handed out stay valid
+  if (slot < 0 || slot >= (int)ws_.size()) throw Error(7, "bad workspace slot");
   if (ws_[slot].bytes < bytes) {
     ws_[slot] = DevBuf();
     ws_[slot] = DevBuf(std::max(bytes, (size_t)256));

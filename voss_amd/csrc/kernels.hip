@@ -555,11 +555,12 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
   uint32_t cur_query = 0xFFFFFFFFu;
   uint32_t since_flush = 0;
   __syncthreads();
-  for (;;) {
-    if (threadIdx.x == 0) lds[kLdsMisc + kMiscItem] = atomicAdd(A.work, 1u);
-    __syncthreads();
-    const uint32_t s = lds[kLdsMisc + kMiscItem];
-    if (s >= A.nseg) break;
+  // Segments are dealt round-robin by workgroup index: the loop bound and the
+  // segment index are wave-uniform, and every thread-0 region below is
+  // followed by a barrier in uniform code (a divergent region at the loop
+  // latch lets the compiler hoist it into a divergent inner loop around the
+  // next barrier).
+  for (uint32_t s = blockIdx.x; s < A.nseg; s += gridDim.x) {
     const SegGeom g = geometry(A, s, MODE);
     if (MODE == kModeBatch && g.query != cur_query) {
       if (cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
@@ -579,7 +580,8 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
         if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
         *reinterpret_cast<uint4*>(dst + 16 * t) = q;
       }
-      continue;
+      __syncthreads();
+      continue;  // loop latch follows a barrier
     }
 
     // checkpoint split (stats mode): at most one bucket boundary per segment
@@ -647,10 +649,11 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
         }
       }
     }
+    __syncthreads();
     if (MODE == kModeStats && ++since_flush == 1024) {
-      __syncthreads();
       flush_hist_stats<BS>(A, lds);
       since_flush = 0;
+      __syncthreads();
     }
   }
   if (MODE == kModeStats) flush_hist_stats<BS>(A, lds);
@@ -771,7 +774,6 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
   const uint32_t wave_bytes = A.seg_bytes / NW;
   uint32_t thr = item.threshold;
-  uint32_t nrec = 0;
   for (int iter = 0; iter < 64; ++iter) {
     if (threadIdx.x == 0) s_best = ~0ull;
     __syncthreads();
@@ -822,22 +824,20 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
         last = l;
       }
       s_best = b;
+      if (b != ~0ull) {
+        unsigned int slot = atomicAdd(R.n_out, 1u);
+        if (slot < R.out_cap) {
+          R.out[slot].position = 30 * g.k0 + (uint32_t)(b >> 20);
+          R.out[slot].gap = (uint32_t)(b & 0xFFFFF);
+          R.out[slot].seg = s;
+        }
+      }
     }
     __syncthreads();
     const unsigned long long b = s_best;
-    __syncthreads();
     if (b == ~0ull) break;
-    const uint32_t v = (uint32_t)(b >> 20), gap = (uint32_t)(b & 0xFFFFF);
-    if (threadIdx.x == 0) {
-      unsigned int slot = atomicAdd(R.n_out, 1u);
-      if (slot < R.out_cap) {
-        R.out[slot].position = 30 * g.k0 + v;
-        R.out[slot].gap = gap;
-        R.out[slot].seg = s;
-      }
-    }
-    thr = gap;
-    ++nrec;
+    thr = (uint32_t)(b & 0xFFFFF);
+    __syncthreads();  // before the next round resets s_best
   }
 }
 
