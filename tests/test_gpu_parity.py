@@ -115,7 +115,7 @@ def _rand_queries(rng, n, lo_exp, hi_exp, wlo, whi):
 def test_batch_ranges_vs_oracle(voss, oracle):
     rng = random.Random(7)
     qs = _rand_queries(rng, 60, 0, 11, 0, 7.3)
-    qs += [(1, 1), (2, 3), (2, 5), (3, 7), (5, 5), (0, 0), (7, 30), (24, 28), (10**10, 10**10), (4, 4)]
+    qs += [(1, 1), (2, 3), (2, 5), (3, 7), (5, 5), (0, 0), (7, 30), (24, 28), (10**10, 10**10), (4, 4), (38, 44), (31, 89), (30, 119), (60, 61), (89, 97), (119, 150)]
     got = voss.stats.batch_ranges([q[0] for q in qs], [q[1] for q in qs])
     for (a, b), g in zip(qs, got):
         o = oracle.range_stats(a, b)

@@ -10,9 +10,19 @@
 
 namespace voss {
 
-// Primes 7..19 are removed by copying a periodic pattern into LDS; sieving
-// proper starts at 23.
-constexpr uint32_t kPresievePeriod = 7u * 11u * 13u * 17u * 19u;  // 323,323 bytes
+// Primes 7..83 are removed by AND-ing six periodic byte patterns into LDS
+// (pattern g has period = product of its primes, in bytes); sieving proper
+// starts at 89.
+constexpr int kPresieveGroups = 6;
+constexpr uint32_t kPresievePeriods[kPresieveGroups] = {
+    7u * 11u * 13u * 17u * 19u,  // 323,323
+    23u * 29u * 31u,             // 20,677
+    37u * 41u * 43u,             // 65,231
+    47u * 53u * 59u,             // 146,969
+    61u * 67u * 71u,             // 290,177
+    73u * 79u * 83u,             // 478,661
+};
+constexpr uint32_t kFirstSievingPrime = 89;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
 constexpr uint32_t kHistBins = 512;   // LDS gap histogram bins (index gap/2)
 constexpr uint32_t kHistCopies = 4;   // copies, one per 16-lane group
@@ -61,10 +71,12 @@ enum KernelMode : int {
 // Arguments of the segment kernel (passed by value).
 struct SieveArgs {
   // resident tables
-  const uint32_t* primes;       // sieving primes >= 23, bits 24..26 = index of p mod 30
+  const uint32_t* primes;       // sieving primes >= 89, bits 24..26 = index of p mod 30
   const uint32_t* pi_tab;       // pi_tab[i] = #sieving primes <= i*kPiTabStep
   uint32_t pi_tab_len;
-  const uint32_t* presieve;     // pattern bytes as u32 words, period kPresievePeriod
+  const uint32_t* presieve;     // pattern bytes as u32 words, kPresieveGroups patterns
+  uint32_t presieve_off[kPresieveGroups];  // word offset of each pattern
+  uint32_t first_word;          // true prime bitmap of bytes 0..3 (values < 120)
   // geometry
   uint64_t k_base;              // sweep: byte index of segment 0
   uint64_t k_end;               // sweep: exclusive end byte
@@ -72,8 +84,8 @@ struct SieveArgs {
   uint32_t seg_bytes;           // S (multiple of 16 * waves * 64... see host)
   uint32_t nseg;                // segments in this launch
   uint32_t seg_base;            // global index of this launch's segment 0
-  uint32_t tier1_end;           // primes [0, tier1_end): one prime per wave, lanes split the segment
-  uint32_t tier2_end;           // primes [tier1_end, tier2_end): lane = (prime, class); rest: lane = prime
+  uint32_t tier1_end;           // primes [0, tier1_end): one prime per wave, lanes split the segment;
+                                // the rest: one prime per lane
   unsigned int* work;           // work counter (zeroed before launch)
   // outputs (mode dependent; unused ones may be null)
   unsigned long long* totals;   // [0] count, [1] pi_4_1 part, [2] pi_3_1 part

@@ -139,11 +139,13 @@ class Engine {
   hipStream_t stream_ = nullptr;
   TuneConfig cfg_;
   // resident tables
-  std::vector<uint32_t> h_primes_;  // sieving primes >= 23 (plain values)
+  std::vector<uint32_t> h_primes_;  // sieving primes >= 89 (plain values)
   uint32_t* d_primes_ = nullptr;
   uint32_t* d_pitab_ = nullptr;
   uint32_t pitab_len_ = 0;
   uint32_t* d_presieve_ = nullptr;
+  uint32_t presieve_off_[kPresieveGroups] = {0};
+  uint32_t first_word_ = 0;
   uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
   KernelTimes times_;
 };

@@ -105,19 +105,42 @@ Engine::Engine() {
   c.tier1_p = env_u32("VOSS_TIER1_P", c.tier1_p);
   set_config(c);
 
-  // presieve pattern: byte j has bit o set iff 30j + W30[o] has no factor in 7..19
-  const size_t pat_bytes = ((size_t)kPresievePeriod + kMaxSegBytes + 4096) & ~(size_t)3;
-  std::vector<uint8_t> pat(pat_bytes);
-  for (size_t j = 0; j < pat_bytes; ++j) {
-    uint8_t b = 0;
-    for (int o = 0; o < 8; ++o) {
-      uint64_t v = 30ull * j + kW30[o];
-      if (v % 7 && v % 11 && v % 13 && v % 17 && v % 19) b |= (uint8_t)(1u << o);
+  // presieve patterns: in pattern g, byte j has bit o set iff 30j + W30[o] has
+  // no factor among the group's primes; each is stored for one period plus a
+  // full segment so a segment never wraps
+  static const uint32_t groups[kPresieveGroups][5] = {
+      {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0}, {37, 41, 43, 0, 0},
+      {47, 53, 59, 0, 0}, {61, 67, 71, 0, 0}, {73, 79, 83, 0, 0}};
+  std::vector<uint32_t> words;
+  for (int gi = 0; gi < kPresieveGroups; ++gi) {
+    presieve_off_[gi] = (uint32_t)words.size();
+    const size_t len = ((size_t)kPresievePeriods[gi] + kMaxSegBytes + 4096 + 3) & ~(size_t)3;
+    std::vector<uint8_t> pat(len);
+    for (size_t j = 0; j < len; ++j) {
+      uint8_t b = 0;
+      for (int o = 0; o < 8; ++o) {
+        const uint64_t v = 30ull * j + kW30[o];
+        bool keep = true;
+        for (int q = 0; q < 5 && groups[gi][q]; ++q) keep = keep && (v % groups[gi][q] != 0);
+        if (keep) b |= (uint8_t)(1u << o);
+      }
+      pat[j] = b;
     }
-    pat[j] = b;
+    const size_t w0 = words.size();
+    words.resize(w0 + len / 4);
+    std::memcpy(words.data() + w0, pat.data(), len);
   }
-  hip_check(hipMalloc(&d_presieve_, pat_bytes), "hipMalloc(presieve)");
-  hip_check(hipMemcpy(d_presieve_, pat.data(), pat_bytes, hipMemcpyHostToDevice), "hipMemcpy(presieve)");
+  hip_check(hipMalloc(&d_presieve_, words.size() * 4), "hipMalloc(presieve)");
+  hip_check(hipMemcpy(d_presieve_, words.data(), words.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(presieve)");
+  // true prime bitmap of bytes 0..3 (values below 120)
+  first_word_ = 0;
+  for (uint32_t byte = 0; byte < 4; ++byte)
+    for (uint32_t o = 0; o < 8; ++o) {
+      const uint32_t v = 30 * byte + kW30[o];
+      bool pr = v >= 2;
+      for (uint32_t d = 2; d * d <= v; ++d) if (v % d == 0) { pr = false; break; }
+      if (pr) first_word_ |= 1u << (8 * byte + o);
+    }
   hip_check(prepare_kernels(lds_bytes_for(kMaxSegBytes)), "prepare_kernels");
   ensure_tables(1000000000000ull);
 }
@@ -146,7 +169,7 @@ void Engine::ensure_tables(uint64_t max_value) {
     if (s[i]) for (uint64_t j = i * i; j <= lim; j += i) s[j] = 0;
   std::vector<uint32_t> entries;
   h_primes_.clear();
-  for (uint64_t p = 23; p <= lim; ++p) {
+  for (uint64_t p = kFirstSievingPrime; p <= lim; ++p) {
     if (!s[p]) continue;
     uint32_t ridx = 0;
     for (uint32_t o = 0; o < 8; ++o) if (kW30[o] == p % 30) ridx = o;
@@ -184,10 +207,11 @@ SieveArgs Engine::base_args(uint32_t nseg) const {
   A.pi_tab = d_pitab_;
   A.pi_tab_len = pitab_len_;
   A.presieve = d_presieve_;
+  for (int k = 0; k < kPresieveGroups; ++k) A.presieve_off[k] = presieve_off_[k];
+  A.first_word = first_word_;
   A.seg_bytes = cfg_.seg_bytes;
   A.nseg = nseg;
   A.tier1_end = tier_index(cfg_.tier1_p);
-  A.tier2_end = tier_index(cfg_.seg_bytes);
   A.large_threshold = 0xFFFFFFFFu;
   return A;
 }

@@ -146,167 +146,162 @@ __device__ void init_tables(uint32_t* lds) {
   }
 }
 
-// Copy the presieved pattern (primes 7..19 removed) into LDS for the segment,
-// zero-filling up to the next 1 KiB so later passes read whole 16-byte chunks.
+// Initialise the segment in LDS as the AND of the presieve patterns (primes
+// 7..83 removed), zero-filling up to the next 1 KiB so later passes read whole
+// 16-byte chunks.  Bytes 0..3 of the global bitmap (values below 120) get
+// their true bits: 1 is not prime, and 7..113 must stay set.
 template <int BS>
 __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
-  const uint32_t off = (uint32_t)(g.k0 % kPresievePeriod);
+  uint32_t off[kPresieveGroups];
+#pragma unroll
+  for (int k = 0; k < kPresieveGroups; ++k) off[k] = (uint32_t)(g.k0 % kPresievePeriods[k]);
   const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
   for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
-    uint32_t src = off + 16 * t;
-    const uint32_t* p = A.presieve + (src >> 2);
-    uint32_t sh = src & 3;
-    uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
-    uint4 v;
-    v.x = __builtin_amdgcn_alignbyte(a1, a0, sh);
-    v.y = __builtin_amdgcn_alignbyte(a2, a1, sh);
-    v.z = __builtin_amdgcn_alignbyte(a3, a2, sh);
-    v.w = __builtin_amdgcn_alignbyte(a4, a3, sh);
-    uint32_t b0 = 16 * t;
+    uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+    for (int k = 0; k < kPresieveGroups; ++k) {
+      const uint32_t src = off[k] + 16 * t;
+      const uint32_t* p = A.presieve + A.presieve_off[k] + (src >> 2);
+      const uint32_t sh = src & 3;
+      const uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
+      v.x &= __builtin_amdgcn_alignbyte(a1, a0, sh);
+      v.y &= __builtin_amdgcn_alignbyte(a2, a1, sh);
+      v.z &= __builtin_amdgcn_alignbyte(a3, a2, sh);
+      v.w &= __builtin_amdgcn_alignbyte(a4, a3, sh);
+    }
+    if (g.k0 < 4 && t == 0) {  // global bytes k0..3 hold the presieved primes themselves
+      const uint32_t sh = 8 * (uint32_t)g.k0;
+      const uint32_t m = 0xFFFFFFFFu >> sh;
+      v.x = (v.x & ~m) | (A.first_word >> sh);
+    }
+    const uint32_t b0 = 16 * t;
     if (b0 + 16 > g.nbytes) {
       uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        uint32_t wb = b0 + 4 * i;
-        uint32_t keep = (wb >= g.nbytes) ? 0u : (g.nbytes - wb >= 4 ? 4u : g.nbytes - wb);
+        const uint32_t wb = b0 + 4 * i;
+        const uint32_t keep = (wb >= g.nbytes) ? 0u : (g.nbytes - wb >= 4 ? 4u : g.nbytes - wb);
         w[i] &= keep >= 4 ? 0xFFFFFFFFu : ((1u << (8 * keep)) - 1u);
       }
       v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
     }
-    if (g.k0 == 0 && t == 0) v.x = (v.x & ~0xFFu) | 0xFEu;  // 1 is not prime; 7..29 are
     *reinterpret_cast<uint4*>(seg8 + b0) = v;
   }
 }
 
-// Cross off every sieving prime >= 23 with p*p <= the segment's top value.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ __forceinline__ void lds_and_at(uint32_t addr, uint32_t m) {
+  __hip_atomic_fetch_and((lds_u32*)(uintptr_t)addr, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// LDS byte address of the segment bitmap.
+__device__ __forceinline__ uint32_t seg_base_addr(uint32_t* lds) {
+  return (uint32_t)(uintptr_t)(lds_u32*)lds + kLdsSeg * 4;
+}
+
+// x mod p for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
+__device__ __forceinline__ uint32_t mod_u32(uint32_t x, uint32_t p) {
+  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)p));
+  int32_t r = (int32_t)(x - q * p);
+  if (r < 0) r += (int32_t)p;
+  if (r >= (int32_t)p) r -= (int32_t)p;
+  return (uint32_t)r;
+}
+
+// Clear bit `bit` of bytes j, j+p, j+2p, ... below `end` (segment byte
+// offsets; base = LDS byte address of the segment).  The byte-in-dword
+// pattern repeats every 4 crossings, so the four dword deltas and masks are
+// loop invariants: one add per crossing plus the ds_and_b32.
+__device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t p, uint32_t end, uint32_t bit) {
+  const uint32_t j0 = j & 3;
+  const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
+  const uint32_t m0 = ~(1u << (j0 * 8 + bit));
+  const uint32_t m1 = ~(1u << (((j0 + p) & 3) * 8 + bit));
+  const uint32_t m2 = ~(1u << (((j0 + 2 * p) & 3) * 8 + bit));
+  const uint32_t m3 = ~(1u << (((j0 + 3 * p) & 3) * 8 + bit));
+  uint32_t a = base + (j & ~3u);
+  const int32_t lim = (int32_t)(base + end) - (int32_t)(j0 + 3 * p);  // a full group fits while a < lim
+  const uint32_t step = 4 * p;
+  while ((int32_t)a < lim) {
+    lds_and_at(a, m0);
+    lds_and_at(a + d1, m1);
+    lds_and_at(a + d2, m2);
+    lds_and_at(a + d3, m3);
+    a += step;
+  }
+  const uint32_t jt = a - base + j0;
+  if (jt < end) lds_and_at(a, m0);
+  if (jt + p < end) lds_and_at(a + d1, m1);
+  if (jt + 2 * p < end) lds_and_at(a + d2, m2);
+}
+
+// First crossing of class c at or after byte k0 of the global bitmap,
+// relative to k0; never below p*p (the reference rule, context.cu:354-388).
+__device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint32_t c, uint32_t rk, uint64_t k0,
+                                                   const uint8_t* tab) {
+  const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
+  int32_t off = (int32_t)cc - (int32_t)rk;
+  if (off < 0) off += (int32_t)p;
+  uint64_t offl = (uint32_t)off;
+  if (30 * k0 < (uint64_t)p * p + 30ull * p) {
+    const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+    const uint64_t fb = (uint64_t)p * m / 30;
+    if (k0 + offl < fb) offl = fb - k0;
+  }
+  return offl;
+}
+
+// Cross off every sieving prime >= 89 with p*p <= the segment's top value.
+//   tier 1 (p < tier1_end-th prime): one prime per wave turn; the 64 lanes
+//     are 8 classes x 8 sub-ranges of the segment;
+//   tier 2 (the rest): one prime per lane, its 8 classes in turn.
 template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
   const uint8_t* tab = reinterpret_cast<const uint8_t*>(lds + kLdsTab);
+  const uint32_t base = seg_base_addr(lds);
   const uint64_t k0 = g.k0;
   const uint32_t nbytes = g.nbytes;
 
   uint64_t top = 30 * (k0 + nbytes) - 1;
   if (g.hi_val < top) top = g.hi_val;
-  uint64_t sq = isqrt_u64(top);
+  const uint64_t sq = isqrt_u64(top);
   uint64_t ti = sq / kPiTabStep + 1;
   if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
   const uint32_t np = A.pi_tab[ti];
-  const uint32_t n1 = min(np, A.tier1_end);  // primes [0, n1): tier 1
+  const uint32_t n1 = min(np, A.tier1_end);
 
-  // ---- tier 1: one prime per wave; lanes = 8 classes x 8 sub-ranges
   {
     const uint32_t c = lane & 7, sr = lane >> 3;
-    const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;  // sub-range length, word aligned
+    const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
     const uint32_t lo = min(sr * sub_len, nbytes);
     const uint32_t hi = min(lo + sub_len, nbytes);
     for (uint32_t i = wave; i < n1; i += NW) {
       const uint32_t e = A.primes[i];
       const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-      const uint32_t rk = mod_u64(k0, p);
-      const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
-      int32_t off = (int32_t)cc - (int32_t)rk;
-      if (off < 0) off += p;
-      uint64_t offl = (uint32_t)off;
-      if (30 * k0 < (uint64_t)p * p + 30ull * p) {  // the p*p rule (rare: low segments)
-        uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
-        uint64_t fb = (uint64_t)p * m / 30;
-        if (k0 + offl < fb) offl = fb - k0;
-      }
+      const uint64_t offl = first_crossing(p, r, c, mod_u64(k0, p), k0, tab);
       if (offl >= hi) continue;
       uint32_t j = (uint32_t)offl;
-      if (j < lo) j += ((lo - j + p - 1) / p) * p;
-      const uint32_t bit = tab[64 + r * 8 + c];
-      // four crossings per turn: byte-in-word pattern repeats every 4 steps
-      const uint32_t m0 = ~(1u << (((j) & 3) * 8 + bit));
-      const uint32_t m1 = ~(1u << (((j + p) & 3) * 8 + bit));
-      const uint32_t m2 = ~(1u << (((j + 2 * p) & 3) * 8 + bit));
-      const uint32_t m3 = ~(1u << (((j + 3 * p) & 3) * 8 + bit));
-      while (j + 3 * p < hi) {
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 3 * p) & ~3u)), m3);
-        j += 4 * p;
+      if (j < lo) {
+        const uint32_t back = mod_u32(lo - j, p);
+        j = lo + (back ? p - back : 0u);
       }
-      if (j < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
-      if (j + p < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
-      if (j + 2 * p < hi) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
+      if (j < hi) cross_class(base, j, p, hi, tab[64 + r * 8 + c]);
     }
   }
-
-  // ---- tier 2: 8 primes per wave turn; lane = (prime, class); walk the segment
-  {
-    const uint32_t c = lane & 7;
-    const uint32_t t2end = min(np, A.tier2_end);
-    for (uint32_t base = n1 + 8 * wave; base < t2end; base += 8 * NW) {
-      const uint32_t i = base + (lane >> 3);
-      if (i >= t2end) continue;
-      const uint32_t e = A.primes[i];
-      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-      const uint32_t rk = mod_u64(k0, p);
-      const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
-      int32_t off = (int32_t)cc - (int32_t)rk;
-      if (off < 0) off += p;
-      uint64_t offl = (uint32_t)off;
-      if (30 * k0 < (uint64_t)p * p + 30ull * p) {
-        uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
-        uint64_t fb = (uint64_t)p * m / 30;
-        if (k0 + offl < fb) offl = fb - k0;
-      }
-      if (offl >= nbytes) continue;
-      uint32_t j = (uint32_t)offl;
-      const uint32_t bit = tab[64 + r * 8 + c];
-      const uint32_t m0 = ~(1u << (((j) & 3) * 8 + bit));
-      const uint32_t m1 = ~(1u << (((j + p) & 3) * 8 + bit));
-      const uint32_t m2 = ~(1u << (((j + 2 * p) & 3) * 8 + bit));
-      const uint32_t m3 = ~(1u << (((j + 3 * p) & 3) * 8 + bit));
-      while (j + 3 * p < nbytes) {
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
-        lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 3 * p) & ~3u)), m3);
-        j += 4 * p;
-      }
-      if (j < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), m0);
-      if (j + p < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + p) & ~3u)), m1);
-      if (j + 2 * p < nbytes) lds_and(reinterpret_cast<uint32_t*>(seg8 + ((j + 2 * p) & ~3u)), m2);
-    }
-  }
-
-  // ---- tier 3: primes >= segment length; lane = prime, at most one crossing per class
-  {
-    const uint32_t t3beg = min(np, A.tier2_end);
-    for (uint32_t base = t3beg + 64 * wave; base < np; base += 64 * NW) {
-      const uint32_t i = base + lane;
-      if (i >= np) continue;
-      const uint32_t e = A.primes[i];
-      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-      const uint32_t rk = mod_u64(k0, p);
-      const uint32_t q30 = p / 30;
-      const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
-      const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
-      const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;
+  for (uint32_t b = n1 + 64 * wave; b < np; b += 64 * NW) {
+    const uint32_t i = b + lane;
+    if (i >= np) continue;
+    const uint32_t e = A.primes[i];
+    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+    const uint32_t rk = mod_u64(k0, p);
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
-        int32_t off = (int32_t)cc - (int32_t)rk;
-        if (off < 0) off += p;
-        uint64_t offl = (uint32_t)off;
-        if (low) {
-          uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
-          uint64_t fb = (uint64_t)p * m / 30;
-          if (k0 + offl < fb) offl = fb - k0;
-        }
-        if (offl < nbytes) {
-          uint32_t j = (uint32_t)offl;
-          uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
-          lds_and(reinterpret_cast<uint32_t*>(seg8 + (j & ~3u)), ~(1u << ((j & 3) * 8 + bit)));
-        }
-      }
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint64_t offl = first_crossing(p, r, c, rk, k0, tab);
+      if (offl < nbytes) cross_class(base, (uint32_t)offl, p, nbytes, tab[64 + r * 8 + c]);
     }
   }
 }
@@ -668,7 +663,9 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
 __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
   __shared__ unsigned long long s_last[1024];
   __shared__ uint32_t s_max[1024];
+  __shared__ uint32_t s_hist[kHistBins];
   const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < kHistBins; i += 1024) s_hist[i] = 0;
   const uint32_t per = (A.nseg + 1023) / 1024;
   const uint32_t lo = min(t * per, A.nseg), hi = min(lo + per, A.nseg);
   auto absval = [&](uint32_t s, uint32_t rel) -> uint64_t {
@@ -722,7 +719,8 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
     const uint64_t F = absval(s, S.first);
     const uint32_t B = (uint32_t)(F - prev);
     const uint32_t h = B >> 1;
-    atomicAdd(A.hist + (h < kGapBins ? h : kGapBins - 1), 1ull);
+    if (h < kHistBins) lds_add(s_hist + h, 1u);
+    else atomicAdd(A.hist + (h < kGapBins ? h : kGapBins - 1), 1ull);
     if (B >= A.large_threshold) {
       unsigned int slot = atomicAdd(A.n_large, 1u);
       if (slot < A.large_cap) { A.large[slot].position = F; A.large[slot].gap = B; A.large[slot].seg = s; }
@@ -743,17 +741,58 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
     A.final_out[0] = prev;
     A.final_out[1] = run;
   }
+  __syncthreads();
+  for (uint32_t i = t; i < kHistBins; i += 1024)
+    if (s_hist[i]) atomicAdd(A.hist + i, (unsigned long long)s_hist[i]);
 }
 
 // ---------------------------------------------------------------- records rescan
-// One workgroup per flagged segment: re-sieve it, then repeatedly find the
-// first gap above the running maximum (which then becomes the new maximum).
+// One workgroup per flagged segment: re-sieve it and list, in one ordered
+// pass, every gap larger than all earlier gaps (starting from the running
+// maximum before the segment).  Pass 1 takes each (wave, step)'s largest
+// gap; thread 0 turns those into exclusive prefix maxima in number order
+// (and checks the gaps between waves); pass 2 re-walks each chunk against
+// its prefix maximum and emits the records.
+
+// Largest gap in a lane's 16-byte chunk (prev = last prime before it, 0 if
+// none), optionally emitting gaps above `run` in order.
+template <bool EMIT>
+__device__ __forceinline__ uint32_t chunk_gaps(uint4 q, uint32_t boff, uint32_t prev, uint32_t run, uint32_t* wfirst,
+                                               const RescanArgs& R, uint64_t k0, uint32_t seg) {
+  uint32_t lm = 0;
+  const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+  for (int wi = 0; wi < 4; ++wi) {
+    uint32_t w = ws[wi];
+    while (w) {
+      const uint32_t b = __builtin_ctz(w);
+      w &= w - 1;
+      const uint32_t v = 30 * (boff + 4 * wi + (b >> 3)) + w30(b & 7);
+      if (prev) {
+        const uint32_t gap = v - prev;
+        lm = max(lm, gap);
+        if (EMIT && gap > run) {
+          run = gap;
+          const unsigned int slot = atomicAdd(R.n_out, 1u);
+          if (slot < R.out_cap) {
+            R.out[slot].position = 30 * k0 + v;
+            R.out[slot].gap = gap;
+            R.out[slot].seg = seg;
+          }
+        }
+      } else if (wfirst) {
+        *wfirst = v;
+      }
+      prev = v;
+    }
+  }
+  return lm;
+}
+
 template <int BS>
 __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   extern __shared__ uint32_t lds[];
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __shared__ unsigned long long s_best;
   init_tables(lds);
   __syncthreads();
   const RescanItem item = R.items[blockIdx.x];
@@ -761,7 +800,7 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   SegGeom g;
   g.query = 0;
   g.k0 = A.k_base + (uint64_t)s * A.seg_bytes;
-  uint64_t rem = A.k_end - g.k0;
+  const uint64_t rem = A.k_end - g.k0;
   g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
   g.lo_val = A.lo_val;
   g.hi_val = A.hi_val;
@@ -771,73 +810,85 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   __syncthreads();
   sieve_segment<BS>(A, lds, g);
   __syncthreads();
+
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
   const uint32_t wave_bytes = A.seg_bytes / NW;
-  uint32_t thr = item.threshold;
-  for (int iter = 0; iter < 64; ++iter) {
-    if (threadIdx.x == 0) s_best = ~0ull;
-    __syncthreads();
-    uint32_t carry = 0, wfirst = 0;
-    unsigned long long best = ~0ull;
-    const uint32_t wbeg = wave * wave_bytes, wend = min(wbeg + wave_bytes, g.nbytes);
-    for (uint32_t base = wbeg; base < wend; base += 1024) {
-      const uint32_t boff = base + 16 * lane;
+  const uint32_t steps = wave_bytes / 1024;  // <= 18
+  uint32_t* stepmax = lds + kLdsMisc + 16;   // NW * steps <= 144 words
+  uint32_t* wfl = lds + kLdsMisc + 176;      // 2 words per wave
+  const uint32_t wbeg = wave * wave_bytes;
+
+  // pass 1: largest gap per (wave, step), first/last prime per wave
+  uint32_t carry = 0, wfirst = 0;
+  for (uint32_t st = 0; st < steps; ++st) {
+    const uint32_t boff = wbeg + 1024 * st + 16 * lane;
+    uint32_t lm = 0;
+    if (wbeg + 1024 * st < g.nbytes) {
       uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
       const uint32_t cb = 8 * boff;
       if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
       uint32_t step_last;
-      uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
+      const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
       carry = max(carry, step_last);
-      const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
-      for (int wi = 0; wi < 4; ++wi) {
-        uint32_t w = ws[wi];
-        while (w) {
-          const uint32_t b = __builtin_ctz(w);
-          w &= w - 1;
-          const uint32_t v = 30 * (boff + 4 * wi + (b >> 3)) + w30(b & 7);
-          if (prev) {
-            if (v - prev > thr && best == ~0ull) best = ((unsigned long long)v << 20) | (v - prev);
-          } else {
-            wfirst = v;
+      lm = chunk_gaps<false>(q, boff, prev, 0, &wfirst, R, g.k0, s);
+    }
+    lm = wave_max(lm);
+    if (lane == 0) stepmax[wave * steps + st] = lm;
+  }
+  wfirst = wave_max(wfirst);
+  if (lane == 0) { wfl[2 * wave] = wfirst; wfl[2 * wave + 1] = carry; }
+  __syncthreads();
+
+  // ordered prefix maxima; gaps between waves are checked here
+  if (threadIdx.x == 0) {
+    uint32_t run = item.threshold, last = 0;
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t f = wfl[2 * w], l = wfl[2 * w + 1];
+      if (l && last) {
+        const uint32_t gap = f - last;
+        if (gap > run) {
+          run = gap;
+          const unsigned int slot = atomicAdd(R.n_out, 1u);
+          if (slot < R.out_cap) {
+            R.out[slot].position = 30 * g.k0 + f;
+            R.out[slot].gap = gap;
+            R.out[slot].seg = s;
           }
-          prev = v;
         }
       }
-    }
-    wfirst = wave_max(wfirst);
-    if (best != ~0ull) atomicMin(&s_best, best);
-    if (lane == 0) {
-      lds[kLdsMisc + kMiscWave + 2 * wave] = wfirst;
-      lds[kLdsMisc + kMiscWave + 2 * wave + 1] = carry;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t last = 0;
-      unsigned long long b = s_best;
-      for (int w = 0; w < NW; ++w) {
-        uint32_t f = lds[kLdsMisc + kMiscWave + 2 * w], l = lds[kLdsMisc + kMiscWave + 2 * w + 1];
-        if (!l) continue;
-        if (last && f - last > thr) {
-          unsigned long long c = ((unsigned long long)f << 20) | (f - last);
-          if (c < b) b = c;
-        }
-        last = l;
+      for (uint32_t st = 0; st < steps; ++st) {
+        const uint32_t m = stepmax[w * steps + st];
+        stepmax[w * steps + st] = run;
+        run = max(run, m);
       }
-      s_best = b;
-      if (b != ~0ull) {
-        unsigned int slot = atomicAdd(R.n_out, 1u);
-        if (slot < R.out_cap) {
-          R.out[slot].position = 30 * g.k0 + (uint32_t)(b >> 20);
-          R.out[slot].gap = (uint32_t)(b & 0xFFFFF);
-          R.out[slot].seg = s;
-        }
-      }
+      if (l) last = l;
     }
-    __syncthreads();
-    const unsigned long long b = s_best;
-    if (b == ~0ull) break;
-    thr = (uint32_t)(b & 0xFFFFF);
-    __syncthreads();  // before the next round resets s_best
+  }
+  __syncthreads();
+
+  // pass 2: emit the records of each chunk against its prefix maximum
+  carry = 0;
+  for (uint32_t st = 0; st < steps; ++st) {
+    if (wbeg + 1024 * st >= g.nbytes) break;  // wave-uniform
+    const uint32_t boff = wbeg + 1024 * st + 16 * lane;
+    uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
+    const uint32_t cb = 8 * boff;
+    if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+    uint32_t step_last;
+    const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
+    carry = max(carry, step_last);
+    const uint32_t lm = chunk_gaps<false>(q, boff, prev, 0, nullptr, R, g.k0, s);
+    // exclusive max of lm over earlier lanes of this step
+    uint32_t incl = lm;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+      if (lane >= (uint32_t)d) incl = max(incl, t);
+    }
+    uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
+    if (lane == 0) excl = 0;
+    const uint32_t run = max(stepmax[wave * steps + st], excl);
+    if (lm > run) chunk_gaps<true>(q, boff, prev, run, nullptr, R, g.k0, s);
   }
 }
 

@@ -11,14 +11,16 @@ _L = _capi._lib
 
 
 def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
-               with_hist: bool = True, rec_cap: int = 4096, large_cap: int = 1 << 16) -> dict:
+               with_hist: bool = True, rec_cap: int = 4096, large_cap: int = 4096,
+               as_numpy: bool = False) -> dict:
     """Every statistic of [2, N] from one GPU sweep.
 
     Returns a dict: prime_count, pi_4_1, pi_4_3, pi_3_1, pi_3_2, last_prime,
     max_gap, hist (list, index = gap; hist[1] counts 2->3), checkpoints (one
     tuple (pi, pi_4_1, pi_4_3, pi_3_1, pi_3_2) per x = j * cp_interval),
-    large_gaps ((position, gap) with gap >= large_threshold, largest first),
-    records ((position, gap) maximal gaps ascending), n_large, n_records.
+    large_gaps ((position, gap) with gap >= large_threshold, largest first, at
+    most large_cap), records ((position, gap) maximal gaps ascending), n_large,
+    n_records.  as_numpy returns hist as a numpy uint64 array (no copy).
     """
     if not isinstance(N, int):
         raise TypeError("N must be int")
@@ -39,11 +41,16 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
         "prime_count": st.prime_count, "pi_4_1": st.pi_4_1, "pi_4_3": st.pi_4_3,
         "pi_3_1": st.pi_3_1, "pi_3_2": st.pi_3_2, "last_prime": st.last_prime,
         "max_gap": st.max_gap, "n_large": st.n_large, "n_records": st.n_records,
-        "hist": list(hist) if with_hist else None,
+        "hist": (_np_view(hist) if as_numpy else list(hist)) if with_hist else None,
         "checkpoints": [tuple(cps[5 * j + f] for f in range(5)) for j in range(ncp)],
         "large_gaps": [(lg[i].position, lg[i].gap) for i in range(nl)],
         "records": [(rec[i].position, rec[i].gap) for i in range(nr)],
     }
+
+
+def _np_view(arr):
+    import numpy as np
+    return np.frombuffer(arr, dtype=np.uint64)
 
 
 def _range_dict(r):
