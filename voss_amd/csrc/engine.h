@@ -25,10 +25,10 @@ struct Error : std::runtime_error {
 
 // Launch configuration chosen by the autotuner (or set by the caller).
 struct TuneConfig {
-  uint32_t seg_bytes = 128 * 1024;  // bytes of bitmap per segment (x30 integers)
+  uint32_t seg_bytes = 144 * 1024;  // bytes of bitmap per segment (x30 integers)
   int block = 1024;                 // threads per workgroup
   int wgs_per_cu = 1;               // persistent workgroups per CU
-  uint32_t tier1_p = 1024;          // primes below this: one prime per wave
+  uint32_t tier1_p = 2048;          // primes below this: one prime per wave
 };
 
 struct GapEntry {

@@ -212,6 +212,7 @@ SieveArgs Engine::base_args(uint32_t nseg) const {
   A.seg_bytes = cfg_.seg_bytes;
   A.nseg = nseg;
   A.tier1_end = tier_index(cfg_.tier1_p);
+  A.tier2_end = tier_index(cfg_.seg_bytes / 4);
   A.large_threshold = 0xFFFFFFFFu;
   return A;
 }
@@ -456,7 +457,9 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   R.pi_3_1 = tot[2];
   R.pi_3_2 = cnt - tot[2] + 2;
   R.last_prime = fin[0];
-  R.max_gap = fin[1];
+  (void)fin[1];
+  for (uint64_t g2 = R.hist.size(); g2-- > 0;)
+    if (R.hist[g2]) { R.max_gap = g2; break; }
   if (ncp) {
     R.checkpoints.resize(5 * ncp);
     uint64_t c = 0, c41 = 0, c31 = 0;

@@ -42,6 +42,11 @@ __device__ __forceinline__ void lds_and(uint32_t* p, uint32_t m) {
 __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+// LDS atomic add at a byte address (ds_add_u32).
+__device__ __forceinline__ void lds_add_at(uint32_t addr, uint32_t v) {
+  __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // k mod p for k < 2^52, p < 2^24, from a Newton-refined FP64 reciprocal.
 __device__ __forceinline__ uint32_t mod_u64(uint64_t k, uint32_t p) {
@@ -93,9 +98,9 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 
 // ---------------------------------------------------------------- LDS layout
 // u32 word offsets into dynamic LDS.
-constexpr uint32_t kLdsHist = 0;                                  // kHistBins*kHistCopies
-constexpr uint32_t kLdsNib = kLdsHist + kHistBins * kHistCopies;  // 256 nibble lists
-constexpr uint32_t kLdsTab = kLdsNib + 256;                       // 3 x 64 bytes = 48 words
+constexpr uint32_t kLdsHist = 0;                                    // kHistBins*kHistCopies
+constexpr uint32_t kLdsByteHist = kLdsHist + kHistBins * kHistCopies; // 256 byte values x copies
+constexpr uint32_t kLdsTab = kLdsByteHist + 256 * kHistCopies;        // 3 x 64 bytes = 48 words
 constexpr uint32_t kLdsMisc = kLdsTab + 48;                       // 256 words
 constexpr uint32_t kLdsSeg = kLdsMisc + 256;                      // segment bytes
 }  // namespace
@@ -118,22 +123,24 @@ struct SegGeom {
   uint32_t query;
 };
 
+// Gaps between consecutive primes inside one bitmap byte x, in order, as
+// 4-bit (gap/2) nibbles; they depend on x alone.
+__device__ __forceinline__ uint32_t internal_gaps(uint32_t x) {
+  uint32_t nib = 0, sh = 0;
+  int prev = -1;
+  for (int o = 0; o < 8; ++o) {
+    if ((x >> o) & 1) {
+      if (prev >= 0) { nib |= ((w30_const(o) - w30_const(prev)) >> 1) << sh; sh += 4; }
+      prev = o;
+    }
+  }
+  return nib;
+}
+
 // Initialise tables in LDS (once per workgroup).
 __device__ void init_tables(uint32_t* lds) {
   const uint32_t tid = threadIdx.x;
   uint8_t* tab = reinterpret_cast<uint8_t*>(lds + kLdsTab);
-  if (tid < 256) {
-    // internal gaps of a byte, in order, as 4-bit (gap/2) nibbles
-    uint32_t x = tid, nib = 0, sh = 0;
-    int prev = -1;
-    for (int o = 0; o < 8; ++o) {
-      if ((x >> o) & 1) {
-        if (prev >= 0) { nib |= ((w30_const(o) - w30_const(prev)) >> 1) << sh; sh += 4; }
-        prev = o;
-      }
-    }
-    lds[kLdsNib + tid] = nib;
-  }
   if (tid < 64) {
     // r = index of p mod 30 in W30, c = class (multiplier m = W30[c] mod 30)
     uint32_t r = tid >> 3, c = tid & 7;
@@ -189,8 +196,6 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     *reinterpret_cast<uint4*>(seg8 + b0) = v;
   }
 }
-
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
 __device__ __forceinline__ void lds_and_at(uint32_t addr, uint32_t m) {
   __hip_atomic_fetch_and((lds_u32*)(uintptr_t)addr, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -254,9 +259,13 @@ __device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint3
 }
 
 // Cross off every sieving prime >= 89 with p*p <= the segment's top value.
-//   tier 1 (p < tier1_end-th prime): one prime per wave turn; the 64 lanes
-//     are 8 classes x 8 sub-ranges of the segment;
-//   tier 2 (the rest): one prime per lane, its 8 classes in turn.
+//   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
+//     lanes are 8 classes x 8 sub-ranges of the segment;
+//   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
+//     unrolled loop;
+//   tier 3 (p >= seg_bytes/4): one prime per lane, at most four crossings per
+//     class, so the per-class setup is kept to a few instructions: class
+//     offsets from two packed 64-bit LDS rows and off = min(cc + p - rk, cc - rk).
 template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
@@ -273,6 +282,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
   const uint32_t np = A.pi_tab[ti];
   const uint32_t n1 = min(np, A.tier1_end);
+  const uint32_t n2 = max(n1, min(np, A.tier2_end));
 
   {
     const uint32_t c = lane & 7, sr = lane >> 3;
@@ -292,9 +302,9 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (j < hi) cross_class(base, j, p, hi, tab[64 + r * 8 + c]);
     }
   }
-  for (uint32_t b = n1 + 64 * wave; b < np; b += 64 * NW) {
+  for (uint32_t b = n1 + 64 * wave; b < n2; b += 64 * NW) {
     const uint32_t i = b + lane;
-    if (i >= np) continue;
+    if (i >= n2) continue;
     const uint32_t e = A.primes[i];
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     const uint32_t rk = mod_u64(k0, p);
@@ -302,6 +312,34 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     for (uint32_t c = 0; c < 8; ++c) {
       const uint64_t offl = first_crossing(p, r, c, rk, k0, tab);
       if (offl < nbytes) cross_class(base, (uint32_t)offl, p, nbytes, tab[64 + r * 8 + c]);
+    }
+  }
+  for (uint32_t b = n2 + 64 * wave; b < np; b += 64 * NW) {
+    const uint32_t i = b + lane;
+    if (i >= np) continue;
+    const uint32_t e = A.primes[i];
+    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+    const uint32_t rk = mod_u64(k0, p);
+    const uint32_t q30 = p / 30;
+    const uint32_t prk = p - rk;
+    const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
+    const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+    const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;  // the p*p rule can bite
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
+      uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
+      if (low) {
+        const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+        const uint64_t fb = (uint64_t)p * m / 30;
+        if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
+      }
+      const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
+      while (off < nbytes) {
+        const uint32_t a = base + off;
+        lds_and_at(a & ~3u, ~(1u << (((a & 3) << 3) | bit)));
+        off += p;
+      }
     }
   }
 }
@@ -400,8 +438,48 @@ __device__ __forceinline__ void note_gap(const SieveArgs& A, uint32_t* lds, uint
   }
 }
 
+// Exact handling of the rare big gaps of a lane's chunk: gaps >= 1024 were
+// counted in the last LDS bin by the fast path and move to their own bin;
+// gaps >= the large-gap threshold are appended to the large-gap list.
+template <int MODE>
+__device__ void chunk_big_gaps(const SieveArgs& A, uint32_t* lds, uint4 q, uint32_t boff, uint32_t prev,
+                               const SegGeom& g, uint32_t copy) {
+  const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+  for (int wi = 0; wi < 4; ++wi) {
+    uint32_t w = ws[wi];
+    while (w) {
+      const uint32_t b = __builtin_ctz(w);
+      w &= w - 1;
+      const uint32_t v = 30 * (boff + 4 * wi + (b >> 3)) + w30(b & 7);
+      if (prev) {
+        const uint32_t gap = v - prev;
+        if (MODE == kModeStats && (gap >> 1) >= kHistBins) {
+          lds_add(lds + kLdsHist + (kHistBins - 1) * kHistCopies + copy, 0xFFFFFFFFu);  // undo the clamp
+          const uint32_t h = gap >> 1;
+          atomicAdd(A.hist + (h < kGapBins ? h : kGapBins - 1), 1ull);
+        }
+        if (MODE == kModeStats && gap >= A.large_threshold) {
+          const unsigned int slot = atomicAdd(A.n_large, 1u);
+          if (slot < A.large_cap) {
+            A.large[slot].position = 30 * g.k0 + v;
+            A.large[slot].gap = gap;
+            A.large[slot].seg = 0;
+          }
+        }
+      }
+      prev = v;
+    }
+  }
+}
+
 // In-order walk of the sieved segment: counts, tallies, gaps; per-wave
-// summaries into LDS misc.  Returns nothing; thread 0 merges afterwards.
+// summaries into LDS misc (thread 0 merges them afterwards).
+//
+// Each lane takes 16 bytes per step (one ds_read_b128; a wave covers 1 KiB).
+// Per byte, branch-free: the gap from the previous prime to the byte's first
+// prime goes to the LDS gap histogram; the gaps between primes of the same
+// byte depend only on the byte value, so a byte with >= 2 primes bumps a
+// histogram of byte values instead, converted to gaps when flushed.
 template <int MODE, int BS>
 __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
   constexpr int NW = BS / 64;
@@ -412,6 +490,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   const uint32_t M41 = (g.k0 & 1) ? 0x99669966u : 0x66996699u;
   const uint32_t M31 = 0x2B2B2B2Bu;
   const bool partial = g.vlo > 0 || g.vhi < 8 * g.nbytes;
+  // gaps at or above this take the exact slow path (stats mode only)
+  const uint32_t slow = (MODE == kModeStats) ? min(A.large_threshold, 2 * kHistBins) : 0xFFFFFFFFu;
+  const uint32_t hbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsHist + copy) * 4;
+  const uint32_t bbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsByteHist + copy) * 4;
 
   uint32_t carry = 0, wfirst = 0;
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
@@ -429,42 +511,38 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
       c41 += popc4m(q, M41);
       c31 += popc4m(q, M31);
       if (split != 0xFFFFFFFFu) {
-        uint4 lo = below_split(q, 8 * boff, split);
+        const uint4 lo = below_split(q, 8 * boff, split);
         lcnt += popc4(lo);
         l41 += popc4m(lo, M41);
         l31 += popc4m(lo, M31);
       }
     }
     if (MODE == kModeCount) continue;
-    const uint32_t lastv = chunk_last(q, boff);
     uint32_t step_last;
-    uint32_t prev = prev_in_step(lastv, lane, step_last);
-    prev = max(prev, carry);
+    const uint32_t prev0 = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
     carry = max(carry, step_last);
-
-    const uint32_t vb = 30 * boff;
+    uint32_t prev = prev0, lm = 0;
     const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-      if (x) {
-        const uint32_t fo = __builtin_ctz(x);
-        const uint32_t vf = vb + 30 * j + w30(fo);
-        if (prev) {
-          note_gap<MODE>(A, lds, vf - prev, vf, g, copy, maxg);
-        } else {
-          wfirst = vf;
-        }
-        uint32_t nib = lds[kLdsNib + x];
-        while (nib) {
-          const uint32_t h = nib & 15;
-          hist_add_lds(lds, h, copy);
-          maxg = max(maxg, 2 * h);
-          nib >>= 4;
-        }
-        prev = vb + 30 * j + w30(31 - __builtin_clz(x));
-      }
+      const uint32_t pc = __builtin_popcount(x);
+      const uint32_t vbj = 30 * (boff + j);
+      const uint32_t vf = vbj + w30(__builtin_ctz(x | 0x100u) & 7);
+      const uint32_t vl = vbj + w30((31 - __builtin_clz(x | 1u)) & 7);
+      const uint32_t gap = vf - prev;
+      const bool ok = (x != 0) & (prev != 0);
+      if (ok) lds_add_at(hbase + min(gap >> 1, kHistBins - 1) * (4 * kHistCopies), 1u);
+      if (pc >= 2) lds_add_at(bbase + x * (4 * kHistCopies), 1u);
+      lm = max(lm, ok ? gap : 0u);
+      wfirst = ((x != 0) & (prev == 0)) ? vf : wfirst;
+      prev = x ? vl : prev;
     }
+    // internal gaps are at most 28; count them in the maximum only coarsely
+    // (an over-estimate only triggers a records rescan of tiny segments)
+    maxg = max(maxg, lm);
+    if (((q.x | q.y | q.z | q.w) != 0) && lm < 28) maxg = max(maxg, 28u);
+    if (lm >= slow) chunk_big_gaps<MODE>(A, lds, q, boff, prev0, g, copy);
   }
   // per-wave reductions
   cnt = wave_sum(cnt);
@@ -487,8 +565,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   }
 }
 
-template <int BS>
-__device__ void flush_hist_stats(const SieveArgs& A, uint32_t* lds) {
+// Flush the LDS histograms: gap bins directly, byte values through their
+// internal gaps.  sink(bin, count) receives gap/2 bins.
+template <int BS, typename Sink>
+__device__ void flush_histograms(uint32_t* lds, Sink sink) {
   for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
     uint32_t s = 0;
 #pragma unroll
@@ -496,22 +576,29 @@ __device__ void flush_hist_stats(const SieveArgs& A, uint32_t* lds) {
       s += lds[kLdsHist + h * kHistCopies + c];
       lds[kLdsHist + h * kHistCopies + c] = 0;
     }
-    if (s) atomicAdd(A.hist + h, (unsigned long long)s);
+    if (s) sink(h, s);
   }
+  for (uint32_t x = threadIdx.x; x < 256; x += BS) {
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kHistCopies; ++c) {
+      s += lds[kLdsByteHist + x * kHistCopies + c];
+      lds[kLdsByteHist + x * kHistCopies + c] = 0;
+    }
+    if (!s) continue;
+    for (uint32_t nib = internal_gaps(x); nib; nib >>= 4) sink(nib & 15, s);
+  }
+}
+
+template <int BS>
+__device__ void flush_hist_stats(const SieveArgs& A, uint32_t* lds) {
+  flush_histograms<BS>(lds, [&](uint32_t h, uint32_t n) { atomicAdd(A.hist + h, (unsigned long long)n); });
 }
 
 template <int BS>
 __device__ void flush_hist_query(const SieveArgs& A, uint32_t* lds, uint32_t qi) {
   unsigned int* qh = A.qhist + (size_t)qi * kQueryBins;
-  for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
-    uint32_t s = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < kHistCopies; ++c) {
-      s += lds[kLdsHist + h * kHistCopies + c];
-      lds[kLdsHist + h * kHistCopies + c] = 0;
-    }
-    if (s) atomicAdd(qh + ((h >= 1 && h <= 256) ? h : 0u), s);
-  }
+  flush_histograms<BS>(lds, [&](uint32_t h, uint32_t n) { atomicAdd(qh + ((h >= 1 && h <= 256) ? h : 0u), n); });
 }
 
 __device__ __forceinline__ SegGeom geometry(const SieveArgs& A, uint32_t s, int mode) {
@@ -546,7 +633,7 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr int NW = BS / 64;
   init_tables(lds);
-  for (uint32_t i = threadIdx.x; i < kHistBins * kHistCopies; i += BS) lds[kLdsHist + i] = 0;
+  for (uint32_t i = threadIdx.x; i < (kHistBins + 256) * kHistCopies; i += BS) lds[kLdsHist + i] = 0;
   uint32_t cur_query = 0xFFFFFFFFu;
   uint32_t since_flush = 0;
   __syncthreads();
