@@ -78,9 +78,18 @@ int voss_amd_nth_prime(uint64_t n, uint64_t* out);
 /* Goldbach partition counts (p1 <= p2) for even n >= 4, one launch for all. */
 int voss_amd_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out);
 
-/* Engine configuration (the autotuner writes the same fields). */
+/* Engine configuration (the autotuner writes the same fields).  kind 0 is
+ * used by sweeps (prime_count, Context, full statistics, bitmaps), kind 1 by
+ * batched ranges; -1 sets both.  seg_bytes: bitmap bytes per LDS segment
+ * (x30 integers; a multiple of 1 KiB x waves, at most 144 KiB); block: 512
+ * or 1024 threads; wgs_per_cu: persistent workgroups per CU (LDS permitting);
+ * tier1_p: primes below it are shared by a whole wave.  At start-up the
+ * engine reads "<sweep|batch> seg block wgs tier1_p" lines from
+ * $VOSS_TUNE_FILE or tuned.cfg next to libvoss.so. */
 int voss_amd_set_config(uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p);
 int voss_amd_get_config(uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p);
+int voss_amd_set_config_for(int kind, uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p);
+int voss_amd_get_config_for(int kind, uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p);
 /* Device time of the last engine call (HIP events on the engine stream). */
 int voss_amd_last_kernel_ms(double* sieve_ms, double* stitch_ms, double* total_ms, uint64_t* segments);
 /* Compute units and architecture name of the device in use. */

@@ -125,6 +125,8 @@ _SIGNATURES = {
     "voss_amd_goldbach_counts": ([_pu64, _u64, _pu64], _int),
     "voss_amd_set_config": ([_u32, _int, _int, _u32], _int),
     "voss_amd_get_config": ([_pu32, _pint, _pint, _pu32], _int),
+    "voss_amd_set_config_for": ([_int, _u32, _int, _int, _u32], _int),
+    "voss_amd_get_config_for": ([_int, _pu32, _pint, _pint, _pu32], _int),
     "voss_amd_last_kernel_ms": ([_P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_double), _pu64], _int),
     "voss_amd_device_info": ([_pint, ctypes.c_char_p, _int], _int),
     "voss_amd_synchronize": ([], _int),

@@ -655,7 +655,8 @@ extern "C" int voss_amd_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t
   });
 }
 
-extern "C" int voss_amd_set_config(uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p) {
+extern "C" int voss_amd_set_config_for(int kind, uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p) {
+  if (kind < -1 || kind > 1) { set_err("kind must be -1 (both), 0 (sweep) or 1 (batch)"); return VOSS_ERR_INVALID_ARG; }
   return guarded([&] {
     Engine& e = eng();
     std::lock_guard<std::mutex> lk(e.mu);
@@ -664,14 +665,30 @@ extern "C" int voss_amd_set_config(uint32_t seg_bytes, int block, int wgs_per_cu
     c.block = block;
     c.wgs_per_cu = wgs_per_cu;
     c.tier1_p = tier1_p;
-    e.set_config(c);
+    e.set_config(c, kind);
+    return VOSS_OK;
+  });
+}
+
+extern "C" int voss_amd_set_config(uint32_t seg_bytes, int block, int wgs_per_cu, uint32_t tier1_p) {
+  return voss_amd_set_config_for(-1, seg_bytes, block, wgs_per_cu, tier1_p);
+}
+
+extern "C" int voss_amd_get_config_for(int kind, uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p) {
+  if (kind < 0 || kind > 1) { set_err("kind must be 0 (sweep) or 1 (batch)"); return VOSS_ERR_INVALID_ARG; }
+  return guarded([&] {
+    const voss::TuneConfig& c = eng().config(kind);
+    if (seg_bytes) *seg_bytes = c.seg_bytes;
+    if (block) *block = c.block;
+    if (wgs_per_cu) *wgs_per_cu = c.wgs_per_cu;
+    if (tier1_p) *tier1_p = c.tier1_p;
     return VOSS_OK;
   });
 }
 
 extern "C" int voss_amd_get_config(uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p) {
   return guarded([&] {
-    const voss::TuneConfig& c = eng().config();
+    const voss::TuneConfig& c = eng().config(0);
     if (seg_bytes) *seg_bytes = c.seg_bytes;
     if (block) *block = c.block;
     if (wgs_per_cu) *wgs_per_cu = c.wgs_per_cu;

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -73,14 +74,33 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// One captured batch pipeline (see Engine::batch_ranges).
+struct BatchPlan {
+  uint32_t qcap = 0, scap = 0;
+  uint32_t seg_bytes = 0, tier1_p = 0;
+  int block = 0, wgs_per_cu = 0;
+  uint64_t table_limit = 0;
+  size_t in_bytes = 0, out_bytes = 0;
+  void* h_in = nullptr;   // pinned: header | queries | segment->query map
+  void* h_out = nullptr;  // pinned: count | first | last | histograms
+  DevBuf d_in, d_sum, d_out, d_work;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  BatchPlan() = default;
+  ~BatchPlan();
+  BatchPlan(const BatchPlan&) = delete;
+  BatchPlan& operator=(const BatchPlan&) = delete;
+};
+
 using ProgressFn = std::function<void(int seg, int total, uint64_t in_seg, uint64_t cumulative)>;
 
 class Engine {
  public:
   static Engine& get();  // initialises the device on first use; throws if none
 
-  const TuneConfig& config() const { return cfg_; }
-  void set_config(const TuneConfig& c);
+  // kind 0: sweeps (prime_count, full statistics, bitmaps); kind 1: batched ranges
+  const TuneConfig& config(int kind = 0) const { return cfgs_[kind & 1]; }
+  void set_config(const TuneConfig& c, int kind = -1);  // -1: both kinds
   int num_cus() const { return ncu_; }
   std::string device_name() const { return name_; }
   hipStream_t stream() const { return stream_; }
@@ -122,9 +142,13 @@ class Engine {
   ~Engine();
   SieveArgs base_args(uint32_t nseg) const;
   int grid_for(uint32_t nseg) const;
+  void load_tune_file();
   uint64_t count_bitmap(uint64_t lo, uint64_t hi);  // wheel-bitmap primes in [max(lo,7), hi]
   // per-segment prime counts of [7, hi] into summaries (count field), returns #segments
   uint32_t segment_counts(uint64_t hi, std::vector<uint32_t>& counts);
+
+  BatchPlan& batch_plan(uint32_t nq, uint32_t nseg);
+  std::vector<std::unique_ptr<BatchPlan>> plans_;
 
   // reusable device workspace, grown on demand
   enum WsSlot {
@@ -137,7 +161,8 @@ class Engine {
   int dev_ = 0, ncu_ = 0;
   std::string name_;
   hipStream_t stream_ = nullptr;
-  TuneConfig cfg_;
+  TuneConfig cfgs_[2];
+  TuneConfig cfg_;  // the configuration of the call in flight
   // resident tables
   std::vector<uint32_t> h_primes_;  // sieving primes >= 89 (plain values)
   uint32_t* d_primes_ = nullptr;

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 
 #include "kernels.h"
 
@@ -98,12 +99,17 @@ Engine::Engine() {
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   ws_.resize(kWsCount_);
 
-  TuneConfig c;
-  c.seg_bytes = env_u32("VOSS_SEG_BYTES", c.seg_bytes);
-  c.block = (int)env_u32("VOSS_BLOCK", (uint32_t)c.block);
-  c.wgs_per_cu = (int)env_u32("VOSS_WGS_PER_CU", (uint32_t)c.wgs_per_cu);
-  c.tier1_p = env_u32("VOSS_TIER1_P", c.tier1_p);
-  set_config(c);
+  load_tune_file();
+  if (std::getenv("VOSS_SEG_BYTES") || std::getenv("VOSS_BLOCK") || std::getenv("VOSS_WGS_PER_CU") ||
+      std::getenv("VOSS_TIER1_P")) {
+    TuneConfig c = cfgs_[0];
+    c.seg_bytes = env_u32("VOSS_SEG_BYTES", c.seg_bytes);
+    c.block = (int)env_u32("VOSS_BLOCK", (uint32_t)c.block);
+    c.wgs_per_cu = (int)env_u32("VOSS_WGS_PER_CU", (uint32_t)c.wgs_per_cu);
+    c.tier1_p = env_u32("VOSS_TIER1_P", c.tier1_p);
+    set_config(c);
+  }
+  cfg_ = cfgs_[0];
 
   // presieve patterns: in pattern g, byte j has bit o set iff 30j + W30[o] has
   // no factor among the group's primes; each is stored for one period plus a
@@ -147,7 +153,43 @@ Engine::Engine() {
 
 Engine::~Engine() {}
 
-void Engine::set_config(const TuneConfig& c) {
+// Tuning file written by the autotuner (voss_amd/autotune.py): lines
+// "<sweep|batch> <seg_bytes> <block> <wgs_per_cu> <tier1_p>".  Looked up in
+// $VOSS_TUNE_FILE, else tuned.cfg next to this library.
+void Engine::load_tune_file() {
+  std::string path;
+  if (const char* env = std::getenv("VOSS_TUNE_FILE")) {
+    path = env;
+  } else {
+    Dl_info info;
+    if (dladdr((void*)&hip_check, &info) && info.dli_fname) {
+      path = info.dli_fname;
+      const size_t slash = path.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuned.cfg";
+    }
+  }
+  FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+  if (!f) return;
+  char kind[32];
+  unsigned seg = 0, t1 = 0;
+  int blk = 0, wgs = 0;
+  while (std::fscanf(f, "%31s %u %d %d %u", kind, &seg, &blk, &wgs, &t1) == 5) {
+    TuneConfig c;
+    c.seg_bytes = seg;
+    c.block = blk;
+    c.wgs_per_cu = wgs;
+    c.tier1_p = t1;
+    try {
+      if (std::string(kind) == "sweep") set_config(c, 0);
+      else if (std::string(kind) == "batch") set_config(c, 1);
+    } catch (const Error&) {
+      // an invalid line keeps the built-in default
+    }
+  }
+  std::fclose(f);
+}
+
+void Engine::set_config(const TuneConfig& c, int kind) {
   if (c.block != 512 && c.block != 1024) throw Error(kErrArg, "block must be 512 or 1024");
   const uint32_t nw = (uint32_t)c.block / 64;
   if (c.seg_bytes == 0 || c.seg_bytes > kMaxSegBytes || c.seg_bytes % (1024 * nw) != 0)
@@ -155,7 +197,8 @@ void Engine::set_config(const TuneConfig& c) {
   if (c.wgs_per_cu < 1 || c.wgs_per_cu > 8) throw Error(kErrArg, "wgs_per_cu must be in [1, 8]");
   if ((uint64_t)lds_bytes_for(c.seg_bytes) * (uint64_t)c.wgs_per_cu > 160 * 1024)
     throw Error(kErrArg, "segment LDS x workgroups per CU exceeds 160 KiB");
-  cfg_ = c;
+  if (kind < 0 || kind == 0) cfgs_[0] = c;
+  if (kind < 0 || kind == 1) cfgs_[1] = c;
 }
 
 void Engine::ensure_tables(uint64_t max_value) {
@@ -235,6 +278,7 @@ DevBuf& Engine::ws(int slot, size_t bytes) {
 
 // ------------------------------------------------------------------ counting
 uint64_t Engine::count_bitmap(uint64_t lo, uint64_t hi) {
+  cfg_ = cfgs_[0];
   if (lo < 7) lo = 7;
   if (hi < lo) return 0;
   ensure_tables(hi);
@@ -301,6 +345,7 @@ SweepStats small_sweep_stats(uint64_t N) {
 
 SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_threshold,
                                const ProgressFn& progress) {
+  cfg_ = cfgs_[0];
   SweepStats R;
   R.hist.assign(100000, 0);
   if (N < 2) throw Error(1, "N must be >= 2");
@@ -487,85 +532,147 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
 }
 
 // ------------------------------------------------------------------ batched ranges
+// A batch is one captured hipGraph per capacity class (queries, segments,
+// both rounded up to powers of two): H2D of the header, queries and
+// segment->query map from pinned staging, a fill kernel for the outputs,
+// the persistent batch sieve (dynamic work-list), the per-query stitch, and
+// D2H of the results.  Replays only rewrite the pinned staging buffers.
+BatchPlan::~BatchPlan() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (h_in) (void)hipHostFree(h_in);
+  if (h_out) (void)hipHostFree(h_out);
+}
+
+static uint32_t pow2_at_least(uint64_t v, uint32_t lo) {
+  uint64_t c = lo;
+  while (c < v) c <<= 1;
+  if (c > 0x80000000ull) throw Error(kErrArg, "batch too large");
+  return (uint32_t)c;
+}
+
+BatchPlan& Engine::batch_plan(uint32_t nq, uint32_t nseg) {
+  const uint32_t qcap = pow2_at_least(nq, 64), scap = pow2_at_least(nseg, 1024);
+  for (auto& p : plans_)
+    if (p->qcap == qcap && p->scap == scap && p->seg_bytes == cfg_.seg_bytes && p->block == cfg_.block &&
+        p->wgs_per_cu == cfg_.wgs_per_cu && p->tier1_p == cfg_.tier1_p && p->table_limit == table_limit_)
+      return *p;
+  if (plans_.size() >= 8) plans_.erase(plans_.begin());
+  plans_.emplace_back(new BatchPlan());
+  BatchPlan& P = *plans_.back();
+  P.qcap = qcap;
+  P.scap = scap;
+  P.seg_bytes = cfg_.seg_bytes;
+  P.block = cfg_.block;
+  P.wgs_per_cu = cfg_.wgs_per_cu;
+  P.tier1_p = cfg_.tier1_p;
+  P.table_limit = table_limit_;
+  P.in_bytes = 64 + (size_t)qcap * sizeof(QueryDesc) + (size_t)scap * 4;
+  P.out_bytes = (size_t)qcap * 24 + (size_t)qcap * kQueryBins * 4;
+  hip_check(hipHostMalloc(&P.h_in, P.in_bytes, hipHostMallocDefault), "hipHostMalloc");
+  hip_check(hipHostMalloc(&P.h_out, P.out_bytes, hipHostMallocDefault), "hipHostMalloc");
+  P.d_in = DevBuf(P.in_bytes);
+  P.d_sum = DevBuf((size_t)scap * sizeof(SegSummary));
+  P.d_out = DevBuf(P.out_bytes);
+  P.d_work = DevBuf(256);
+  uint8_t* din = P.d_in.as<uint8_t>();
+  const uint32_t* d_hdr = reinterpret_cast<const uint32_t*>(din);  // [0] nq, [1] nseg
+  const QueryDesc* d_q = reinterpret_cast<const QueryDesc*>(din + 64);
+  const uint32_t* d_sq = reinterpret_cast<const uint32_t*>(din + 64 + (size_t)qcap * sizeof(QueryDesc));
+  unsigned long long* d_qc = P.d_out.as<unsigned long long>();
+  unsigned int* d_qh = reinterpret_cast<unsigned int*>(P.d_out.as<uint8_t>() + (size_t)qcap * 24);
+
+  SieveArgs A = base_args(scap);
+  A.work = P.d_work.as<unsigned int>();
+  A.nseg_dev = d_hdr + 1;
+  A.summaries = P.d_sum.as<SegSummary>();
+  A.queries = d_q;
+  A.seg_query = d_sq;
+  A.qhist = d_qh;
+  A.qcount = d_qc;
+  BatchStitchArgs B;
+  B.queries = d_q;
+  B.summaries = P.d_sum.as<SegSummary>();
+  B.nq = qcap;
+  B.nq_dev = d_hdr;
+  B.seg_bytes = cfg_.seg_bytes;
+  B.qhist = d_qh;
+  B.first = d_qc + qcap;
+  B.last = d_qc + 2 * qcap;
+
+  hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hip_check(hipMemcpyAsync(P.d_in.p, P.h_in, P.in_bytes, hipMemcpyHostToDevice, stream_), "capture H2D");
+  hip_check(launch_zero(P.d_out.as<uint32_t>(), P.out_bytes / 4, stream_), "capture zero");
+  hip_check(launch_zero(P.d_work.as<uint32_t>(), 64, stream_), "capture zero");
+  hip_check(launch_sieve(kModeBatch, cfg_.block, A, grid_for(scap), stream_), "capture sieve");
+  hip_check(launch_batch_stitch(B, qcap, stream_), "capture stitch");
+  hip_check(hipMemcpyAsync(P.h_out, P.d_out.p, P.out_bytes, hipMemcpyDeviceToHost, stream_), "capture D2H");
+  hip_check(hipStreamEndCapture(stream_, &P.graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  return P;
+}
+
 void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeOut* out) {
+  cfg_ = cfgs_[1];
   const uint32_t S = cfg_.seg_bytes;
-  std::vector<QueryDesc> qs;
-  std::vector<uint32_t> qmap;  // device query -> caller index
+  std::vector<uint32_t> order;  // caller index, costliest segments first
   uint64_t bmax = 7;
-  uint64_t total = 0;
   for (size_t i = 0; i < n; ++i) {
     out[i] = RangeOut();
     const uint64_t lo = std::max<uint64_t>(a[i], 7);
     if (b[i] < lo) continue;
-    QueryDesc Q;
-    Q.a = lo;
-    Q.b = b[i];
-    Q.k_lo = lo / 30;
-    Q.k_hi = b[i] / 30 + 1;
-    const uint64_t ns = (Q.k_hi - Q.k_lo + S - 1) / S;
-    Q.seg_first = (uint32_t)total;
-    Q.nseg = (uint32_t)ns;
-    total += ns;
-    if (total > 0x7FFFFFF0ull) throw Error(kErrArg, "batch too large");
+    order.push_back((uint32_t)i);
     bmax = std::max(bmax, b[i]);
-    qs.push_back(Q);
-    qmap.push_back((uint32_t)i);
   }
-  const uint32_t nq = (uint32_t)qs.size();
-  const uint32_t nseg = (uint32_t)total;
+  // per-segment cost grows with sqrt(b): deal the expensive queries first
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return b[x] > b[y]; });
+  const uint32_t nq = (uint32_t)order.size();
   times_ = KernelTimes();
   if (nq) {
     ensure_tables(bmax);
-    std::vector<uint32_t> segq(nseg);
-    for (uint32_t q = 0; q < nq; ++q)
-      for (uint32_t i = 0; i < qs[q].nseg; ++i) segq[qs[q].seg_first + i] = q;
-    DevBuf& d_q = ws(kWsQueries, (size_t)nq * sizeof(QueryDesc));
-    DevBuf& d_sq = ws(kWsSegQuery, (size_t)nseg * 4);
-    DevBuf& d_sum = ws(kWsSummaries, (size_t)nseg * sizeof(SegSummary));
-    DevBuf& d_qh = ws(kWsQHist, (size_t)nq * kQueryBins * 4);
-    DevBuf& d_qc = ws(kWsQCount, (size_t)nq * 8 * 3);
-    DevBuf& d_work = ws(kWsWork, 64);
-    hip_check(hipMemcpyAsync(d_q.p, qs.data(), (size_t)nq * sizeof(QueryDesc), hipMemcpyHostToDevice, stream_), "H2D");
-    hip_check(hipMemcpyAsync(d_sq.p, segq.data(), (size_t)nseg * 4, hipMemcpyHostToDevice, stream_), "H2D");
-    hip_check(hipMemsetAsync(d_qh.p, 0, (size_t)nq * kQueryBins * 4, stream_), "memset");
-    hip_check(hipMemsetAsync(d_qc.p, 0, (size_t)nq * 8 * 3, stream_), "memset");
-    hip_check(hipMemsetAsync(d_work.p, 0, 64, stream_), "memset");
-    SieveArgs A = base_args(nseg);
-    A.work = d_work.as<unsigned int>();
-    A.summaries = d_sum.as<SegSummary>();
-    A.queries = d_q.as<QueryDesc>();
-    A.seg_query = d_sq.as<uint32_t>();
-    A.qhist = d_qh.as<unsigned int>();
-    A.qcount = d_qc.as<unsigned long long>();
-    BatchStitchArgs B;
-    B.queries = d_q.as<QueryDesc>();
-    B.summaries = d_sum.as<SegSummary>();
-    B.nq = nq;
-    B.seg_bytes = S;
-    B.qhist = d_qh.as<unsigned int>();
-    B.first = d_qc.as<unsigned long long>() + nq;
-    B.last = d_qc.as<unsigned long long>() + 2 * nq;
-    Event e0, e1, e2;
+    uint64_t total = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+      const uint64_t lo = std::max<uint64_t>(a[order[q]], 7);
+      total += ((b[order[q]] / 30 + 1) - lo / 30 + S - 1) / S;
+    }
+    if (total > 0x40000000ull) throw Error(kErrArg, "batch too large");
+    const uint32_t nseg = (uint32_t)total;
+    BatchPlan& P = batch_plan(nq, nseg);
+    uint8_t* hin = static_cast<uint8_t*>(P.h_in);
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(hin);
+    QueryDesc* hq = reinterpret_cast<QueryDesc*>(hin + 64);
+    uint32_t* hsq = reinterpret_cast<uint32_t*>(hin + 64 + (size_t)P.qcap * sizeof(QueryDesc));
+    hdr[0] = nq;
+    hdr[1] = nseg;
+    uint32_t seg = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+      const uint32_t i = order[q];
+      QueryDesc& Q = hq[q];
+      Q.a = std::max<uint64_t>(a[i], 7);
+      Q.b = b[i];
+      Q.k_lo = Q.a / 30;
+      Q.k_hi = b[i] / 30 + 1;
+      Q.seg_first = seg;
+      Q.nseg = (uint32_t)((Q.k_hi - Q.k_lo + S - 1) / S);
+      for (uint32_t k = 0; k < Q.nseg; ++k) hsq[seg + k] = q;
+      seg += Q.nseg;
+    }
+    Event e0, e1;
     hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
-    hip_check(launch_sieve(kModeBatch, cfg_.block, A, grid_for(nseg), stream_), "launch(batch)");
+    hip_check(hipGraphLaunch(P.exec, stream_), "hipGraphLaunch");
     hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
-    hip_check(launch_batch_stitch(B, stream_), "launch(batch_stitch)");
-    hip_check(hipEventRecord(e2.e, stream_), "hipEventRecord");
-    std::vector<unsigned long long> qc((size_t)nq * 3);
-    std::vector<uint32_t> qh((size_t)nq * kQueryBins);
-    hip_check(hipMemcpyAsync(qc.data(), d_qc.p, (size_t)nq * 24, hipMemcpyDeviceToHost, stream_), "D2H");
-    hip_check(hipMemcpyAsync(qh.data(), d_qh.p, (size_t)nq * kQueryBins * 4, hipMemcpyDeviceToHost, stream_), "D2H");
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     times_.sieve_ms = elapsed(e0, e1);
-    times_.stitch_ms = elapsed(e1, e2);
-    times_.total_ms = elapsed(e0, e2);
+    times_.total_ms = times_.sieve_ms;
     times_.segments = nseg;
+    const unsigned long long* qc = static_cast<const unsigned long long*>(P.h_out);
+    const uint32_t* qh = reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(P.h_out) + (size_t)P.qcap * 24);
     for (uint32_t q = 0; q < nq; ++q) {
-      RangeOut& o = out[qmap[q]];
+      RangeOut& o = out[order[q]];
       o.count = qc[q];
-      o.first = qc[nq + q];
-      o.last = qc[2 * nq + q];
-      std::memcpy(o.hist, qh.data() + (size_t)q * kQueryBins, kQueryBins * 4);
+      o.first = qc[P.qcap + q];
+      o.last = qc[2 * P.qcap + q];
+      std::memcpy(o.hist, qh + (size_t)q * kQueryBins, kQueryBins * 4);
     }
   }
   // seeds 2, 3, 5 (not in the wheel bitmap) for queries that include them
@@ -588,6 +695,7 @@ void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeO
 
 // ------------------------------------------------------------------ bitmap + list
 uint64_t Engine::bitmap_range(uint64_t lo, uint64_t hi, DevBuf& out, uint64_t* k_base_out) {
+  cfg_ = cfgs_[0];
   if (lo < 7) lo = 7;
   const uint32_t S = cfg_.seg_bytes;
   const uint64_t k_base = lo / 30, k_end = hi / 30 + 1;

@@ -117,6 +117,7 @@ void Engine::next_prev_batch(const uint64_t* x, size_t n, uint64_t* out, int dir
 }
 
 uint32_t Engine::segment_counts(uint64_t hi, std::vector<uint32_t>& counts) {
+  cfg_ = cfgs_[0];
   ensure_tables(hi);
   const uint32_t S = cfg_.seg_bytes;
   const uint64_t k_end = hi / 30 + 1;
@@ -162,7 +163,7 @@ uint64_t Engine::nth_prime(uint64_t n) {
     cum += counts[s];
   }
   if (s == nseg) throw Error(kErrInternal, "internal: could not locate nth prime");
-  const uint64_t S = cfg_.seg_bytes;
+  const uint64_t S = cfgs_[0].seg_bytes;
   const uint64_t lo = std::max<uint64_t>(30 * (uint64_t)s * S, 7);
   const uint64_t top = std::min<uint64_t>(30 * ((uint64_t)s + 1) * S - 1, hi);
   bool of = false;

@@ -38,6 +38,7 @@ struct BatchStitchArgs {
   const QueryDesc* queries;
   const SegSummary* summaries;
   uint32_t nq;
+  const uint32_t* nq_dev;  // if set, the query count is read here
   uint32_t seg_bytes;
   unsigned int* qhist;
   unsigned long long* first;
@@ -53,7 +54,9 @@ hipError_t prepare_kernels(uint32_t lds_bytes);
 hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipStream_t st);
 hipError_t launch_stitch(const StitchArgs& A, hipStream_t st);
 hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st);
-hipError_t launch_batch_stitch(const BatchStitchArgs& A, hipStream_t st);
+hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStream_t st);
+// Zero n 32-bit words (graph-safe fill, no memset node).
+hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st);
 
 // Bitmap -> ascending list of primes, in blocks of 4096 bitmap bytes:
 // counts[b] = primes in block b; out[block_off[b] + i] = i-th prime of block b.

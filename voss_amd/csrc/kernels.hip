@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -642,7 +644,17 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
   // followed by a barrier in uniform code (a divergent region at the loop
   // latch lets the compiler hoist it into a divergent inner loop around the
   // next barrier).
-  for (uint32_t s = blockIdx.x; s < A.nseg; s += gridDim.x) {
+  const uint32_t nseg = (MODE == kModeBatch && A.nseg_dev) ? *A.nseg_dev : A.nseg;
+  uint32_t s = blockIdx.x;
+  if (MODE == kModeBatch) {
+    // dynamic work-list: thread 0 takes the next segment; the item is read
+    // after one barrier and the slot reused only after the end-of-iteration
+    // barrier, so the loop stays wave-uniform
+    if (threadIdx.x == 0) lds[kLdsMisc + kMiscItem] = atomicAdd(A.work, 1u);
+    __syncthreads();
+    s = lds[kLdsMisc + kMiscItem];
+  }
+  while (s < nseg) {
     const SegGeom g = geometry(A, s, MODE);
     if (MODE == kModeBatch && g.query != cur_query) {
       if (cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
@@ -663,7 +675,8 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
         *reinterpret_cast<uint4*>(dst + 16 * t) = q;
       }
       __syncthreads();
-      continue;  // loop latch follows a barrier
+      s += gridDim.x;  // bitmap mode is a sweep: static round-robin
+      continue;
     }
 
     // checkpoint split (stats mode): at most one bucket boundary per segment
@@ -736,6 +749,13 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       flush_hist_stats<BS>(A, lds);
       since_flush = 0;
       __syncthreads();
+    }
+    if (MODE == kModeBatch) {
+      if (threadIdx.x == 0) lds[kLdsMisc + kMiscItem] = atomicAdd(A.work, 1u);
+      __syncthreads();
+      s = lds[kLdsMisc + kMiscItem];
+    } else {
+      s += gridDim.x;
     }
   }
   if (MODE == kModeStats) flush_hist_stats<BS>(A, lds);
@@ -984,7 +1004,8 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
 // query histogram) and report first and last prime.
 __global__ void batch_stitch_kernel(BatchStitchArgs A) {
   const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
-  if (qi >= A.nq) return;
+  const uint32_t nq = A.nq_dev ? *A.nq_dev : A.nq;
+  if (qi >= nq) return;
   const QueryDesc Q = A.queries[qi];
   uint64_t first = 0, last = 0;
   unsigned int* qh = A.qhist + (size_t)qi * kQueryBins;
@@ -1115,9 +1136,21 @@ hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int
   return hipGetLastError();
 }
 
-hipError_t launch_batch_stitch(const BatchStitchArgs& A, hipStream_t st) {
-  if (A.nq == 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_stitch_kernel, dim3((A.nq + 255) / 256), dim3(256), 0, st, A);
+hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStream_t st) {
+  if (nq_cap == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_stitch_kernel, dim3((nq_cap + 255) / 256), dim3(256), 0, st, A);
+  return hipGetLastError();
+}
+
+__global__ void zero_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st) {
+  if (!n) return hipSuccess;
+  const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

@@ -65,13 +65,45 @@ def range_stats(a: int, b: int) -> dict:
     return _range_dict(out)
 
 
-def batch_ranges(a: list, b: list) -> list:
-    """range_stats for many queries in one persistent launch."""
+_RANGE_DTYPE = None
+
+
+def _range_dtype():
+    global _RANGE_DTYPE
+    if _RANGE_DTYPE is None:
+        import numpy as np
+        _RANGE_DTYPE = np.dtype([("count", "<u8"), ("first", "<u8"), ("last", "<u8"),
+                                 ("hist", "<u4", (257,)), ("pad", "<u4")])
+    return _RANGE_DTYPE
+
+
+def _u64_arg(v):
+    try:
+        import numpy as np
+        if isinstance(v, np.ndarray):
+            arr = np.ascontiguousarray(v, dtype=np.uint64)
+            return arr, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    except ImportError:
+        pass
+    arr = _capi.u64_array(v)
+    return arr, arr
+
+
+def batch_ranges(a, b, as_numpy: bool = False):
+    """range_stats for many queries in one persistent launch.
+
+    as_numpy returns a numpy structured array (fields count, first, last,
+    hist[257]) viewing the results without per-element conversion."""
     n = len(a)
     if len(b) != n:
         raise ValueError("a and b must have the same length")
     out = (_capi.AmdRange * max(1, n))()
-    _capi._check(_L.voss_amd_batch_ranges(_capi.u64_array(a), _capi.u64_array(b), n, out), _ERRMAP)
+    ka, pa = _u64_arg(a)
+    kb, pb = _u64_arg(b)
+    _capi._check(_L.voss_amd_batch_ranges(pa, pb, n, out), _ERRMAP)
+    if as_numpy:
+        import numpy as np
+        return np.frombuffer(out, dtype=_range_dtype(), count=n)
     return [_range_dict(out[i]) for i in range(n)]
 
 
@@ -119,14 +151,19 @@ def goldbach_counts(ns: list) -> list:
     return list(out[:k])
 
 
-def set_config(seg_bytes: int, block: int = 1024, wgs_per_cu: int = 1, tier1_p: int = 1024) -> None:
-    _capi._check(_L.voss_amd_set_config(seg_bytes, block, wgs_per_cu, tier1_p), _ERRMAP)
+KIND = {"both": -1, "sweep": 0, "batch": 1}
 
 
-def get_config() -> dict:
+def set_config(seg_bytes: int, block: int = 1024, wgs_per_cu: int = 1, tier1_p: int = 2048,
+               kind: str = "both") -> None:
+    """Engine launch configuration for sweeps, batched ranges, or both."""
+    _capi._check(_L.voss_amd_set_config_for(KIND[kind], seg_bytes, block, wgs_per_cu, tier1_p), _ERRMAP)
+
+
+def get_config(kind: str = "sweep") -> dict:
     s, b, w, t = ctypes.c_uint32(), ctypes.c_int(), ctypes.c_int(), ctypes.c_uint32()
-    _capi._check(_L.voss_amd_get_config(ctypes.byref(s), ctypes.byref(b), ctypes.byref(w),
-                                        ctypes.byref(t)), _ERRMAP)
+    _capi._check(_L.voss_amd_get_config_for(KIND[kind], ctypes.byref(s), ctypes.byref(b), ctypes.byref(w),
+                                            ctypes.byref(t)), _ERRMAP)
     return {"seg_bytes": s.value, "block": b.value, "wgs_per_cu": w.value, "tier1_p": t.value}
 
 
