@@ -69,14 +69,84 @@ int voss_amd_count_range(uint64_t a, uint64_t b, uint64_t* out);
 
 /* GPU primality of n values (deterministic for every u64); out[i] = 0/1. */
 int voss_amd_is_prime_batch(const uint64_t* x, uint64_t n, uint8_t* out);
-/* GPU next/prev prime of n values (same semantics as voss_primes_next/prev;
- * prev of x <= 2 gives 0 in out and rc VOSS_OK for the batch). */
+/* GPU next/prev prime of n values (same semantics as voss_primes_next/prev,
+ * except that where those fail the batch writes 0 and still returns VOSS_OK:
+ * prev of x <= 2, next of x >= 2^64 - 59). */
 int voss_amd_next_prime_batch(const uint64_t* x, uint64_t n, uint64_t* out);
 int voss_amd_prev_prime_batch(const uint64_t* x, uint64_t n, uint64_t* out);
 /* nth prime without the reference's 5e7 cap (n <= 1e12). */
 int voss_amd_nth_prime(uint64_t n, uint64_t* out);
 /* Goldbach partition counts (p1 <= p2) for even n >= 4, one launch for all. */
 int voss_amd_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out);
+
+/* ---------------------------------------------------------------- serving
+ * A continuous-batching request server (one per process).  Requests of mixed
+ * kinds are admitted in arrival order into whichever of `slots` device
+ * streams is free, as many per tick as max_batch and the device-memory budget
+ * allow (working bytes in flight <= min(hbm_budget, free HBM - hbm_reserve)),
+ * so work that arrives while other ticks run overlaps on the device.  The
+ * sieving-prime table, a prefix-count index for nth (n <= nth_max) and the
+ * wheel bitmap of [0, goldbach_max] stay resident.  Each response equals what
+ * the plain entry points return for the same request. */
+enum {
+    VOSS_AMD_REQ_RANGE = 0,    /* a, b: count (value), first, last, hist as voss_amd_range_t */
+    VOSS_AMD_REQ_NTH = 1,      /* a = n (1-based): value = p_n */
+    VOSS_AMD_REQ_NEXT = 2,     /* a = x: value = smallest prime > x */
+    VOSS_AMD_REQ_PREV = 3,     /* a = x: value = largest prime < x, 0 when x <= 2 */
+    VOSS_AMD_REQ_IS_PRIME = 4, /* x[0..n): value bit i = x[i] is prime */
+    VOSS_AMD_REQ_GOLDBACH = 5  /* a = even n >= 4: value = #{p1 <= p2 : p1 + p2 = n} */
+};
+
+typedef struct {
+    uint32_t kind;
+    uint32_t n;       /* is_prime: values in x (<= 64) */
+    uint64_t a, b;
+    uint64_t x[64];
+} voss_amd_request_t;
+
+typedef struct {
+    int32_t status;   /* VOSS_OK or the code the plain entry point would return */
+    uint32_t pad;
+    uint64_t value;
+    uint64_t first, last;
+    uint32_t hist[257];
+    uint32_t pad2;
+    double latency_ms; /* completion minus arrival */
+    uint64_t tick;     /* serial number of the batch that carried the request */
+} voss_amd_response_t;
+
+typedef struct {
+    int32_t slots;         /* concurrent device streams (1..8); default 4 */
+    uint32_t max_batch;    /* requests per tick; default 1024 */
+    uint64_t hbm_reserve;  /* bytes left free on the device; default 1 GiB */
+    uint64_t hbm_budget;   /* working bytes in flight; default 8 GiB */
+    uint64_t nth_max;      /* default 10^12 */
+    uint64_t goldbach_max; /* default 10^8 */
+    int32_t plain;         /* 1: one request per tick on one stream (the per-query path) */
+    int32_t pad;
+} voss_amd_server_config_t;
+
+typedef struct {
+    double wall_s;        /* first arrival to last completion */
+    double qps;           /* requests / wall_s */
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double warmup_s;      /* building the resident index and bitmap */
+    uint64_t ticks, max_tick, deferred_by_memory, peak_inflight_bytes;
+} voss_amd_serve_stats_t;
+
+/* Fill cfg with the defaults. */
+int voss_amd_server_default_config(voss_amd_server_config_t* cfg);
+/* Start (or restart) the server; NULL cfg = defaults.  Builds the resident state. */
+int voss_amd_server_start(const voss_amd_server_config_t* cfg);
+int voss_amd_server_stop(void);
+/* Enqueue one request (arrival = now); *ticket identifies it for _wait. */
+int voss_amd_server_submit(const voss_amd_request_t* req, uint64_t* ticket);
+/* Block until the ticket's response is ready (each ticket is waited once). */
+int voss_amd_server_wait(uint64_t ticket, voss_amd_response_t* out);
+/* Open-loop replay: request i is submitted arrival_s[i] seconds after the
+ * start; returns every response and the latency/throughput summary. */
+int voss_amd_server_replay(const voss_amd_request_t* reqs, const double* arrival_s, uint64_t n,
+                           voss_amd_response_t* out, voss_amd_serve_stats_t* stats);
 
 /* Engine configuration (the autotuner writes the same fields).  kind 0 is
  * used by sweeps (prime_count, Context, full statistics, bitmaps), kind 1 by

@@ -1,0 +1,105 @@
+"""The C ABI without a GPU: libvoss.so loads, exports every function the
+headers in include/voss declare, the Python binding covers them all, argument
+checks fire before any device work, and device entry points fail loudly
+(VOSS_ERR_NO_CUDA) instead of falling back to the CPU."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADERS = sorted((ROOT / "include" / "voss").glob("*.h"))
+DECL = re.compile(r"^[A-Za-z_][\w\s\*]*?\b(voss_\w+)\s*\(", re.M)
+
+
+def declared():
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        text = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith(("#", "typedef")))
+        names.update(DECL.findall(text))
+    return names
+
+
+def test_headers_declare_the_reference_surface():
+    names = declared()
+    assert len(names) >= 60
+    for must in ("voss_primes_prime_count", "voss_primes_ctx_new", "voss_primes_goldbach_test_upto",
+                 "voss_primes_nth", "voss_numbers_phi", "voss_amd_full_stats", "voss_amd_batch_ranges",
+                 "voss_amd_server_replay"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(ROOT / "voss_amd" / "libvoss.so"))
+    missing = [n for n in sorted(declared()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_every_symbol(voss):
+    from voss_amd import _capi
+    assert declared() <= set(_capi._SIGNATURES), sorted(declared() - set(_capi._SIGNATURES))
+
+
+def _no_device(voss):
+    try:
+        voss.stats.device_info()
+    except voss.VossError:
+        return True
+    return False
+
+
+def test_argument_checks_need_no_device(voss):
+    from voss_amd import _capi
+    L = _capi._lib
+    out = ctypes.c_uint64()
+    assert L.voss_primes_prime_count(1, ctypes.byref(out)) == _capi.VOSS_ERR_INVALID_N
+    assert L.voss_primes_prime_count(10**14 + 1, ctypes.byref(out)) == _capi.VOSS_ERR_OUT_OF_RANGE
+    assert L.voss_primes_nth(0, ctypes.byref(out)) == _capi.VOSS_ERR_INVALID_N
+    assert L.voss_primes_goldbach_count(7, ctypes.byref(out)) == _capi.VOSS_ERR_INVALID_N
+    assert L.voss_amd_nth_prime(10**12 + 1, ctypes.byref(out)) == _capi.VOSS_ERR_OUT_OF_RANGE
+    assert L.voss_amd_server_submit(None, ctypes.byref(out)) == _capi.VOSS_ERR_INVALID_ARG
+    # host-only entry points work without a device
+    assert L.voss_numbers_phi(36, ctypes.byref(out)) == 0 and out.value == 12
+    assert L.voss_primes_next(100, ctypes.byref(out)) == 0 and out.value == 101
+    cfg = _capi.AmdServerConfig()
+    assert L.voss_amd_server_default_config(ctypes.byref(cfg)) == 0
+    assert (cfg.slots, cfg.max_batch, cfg.nth_max, cfg.goldbach_max) == (4, 1024, 10**12, 10**8)
+
+
+def test_device_paths_fail_loudly_without_a_device(voss):
+    if not _no_device(voss):
+        pytest.skip("a device is present")
+    from voss_amd import _capi
+    L = _capi._lib
+    out = ctypes.c_uint64()
+    assert L.voss_primes_prime_count(10**9, ctypes.byref(out)) == _capi.VOSS_ERR_NO_CUDA
+    assert L.voss_amd_server_start(None) == _capi.VOSS_ERR_NO_CUDA
+    with pytest.raises(voss.VossError):
+        voss.stats.range_stats(10, 100)
+    with pytest.raises(voss.VossError):
+        voss.primes.prime_count(10**6)
+
+
+def test_serving_trace_is_deterministic(voss):
+    from voss_amd import serve
+    r1, a1 = serve.make_trace(4000, rate=2000.0, seed=1)
+    r2, a2 = serve.make_trace(4000, rate=2000.0, seed=1)
+    assert np.array_equal(r1, r2) and np.array_equal(a1, a2)
+    share = np.bincount(r1["kind"], minlength=6) / len(r1)
+    assert abs(share[serve.RANGE] - 0.5) < 0.03
+    assert abs(share[[serve.NTH, serve.NEXT, serve.PREV]].sum() - 0.2) < 0.03
+    assert abs(share[serve.IS_PRIME] - 0.2) < 0.03
+    assert abs(share[serve.GOLDBACH] - 0.1) < 0.03
+    assert abs(len(a1) / a1[-1] - 2000.0) < 150
+    g = r1["a"][r1["kind"] == serve.GOLDBACH]
+    assert (g % 2 == 0).all() and g.min() >= 4 and g.max() <= 10**8
+    rq = r1[r1["kind"] == serve.RANGE]
+    w = rq["b"] - rq["a"] + 1
+    assert rq["a"].min() >= 10**6 and rq["a"].max() <= 10**13 and w.min() >= 10**3 and w.max() <= 10**9
+    nnp = r1["a"][np.isin(r1["kind"], [serve.NTH, serve.NEXT, serve.PREV])]
+    assert nnp.min() >= 1 and nnp.max() <= 10**12
+    assert (r1["n"][r1["kind"] == serve.IS_PRIME] == 64).all()

@@ -154,6 +154,15 @@ def test_goldbach(voss, oracle):
     assert r["status"] == "no_counterexample_found_upto_N"
 
 
+def test_goldbach_counts_many(voss, oracle):
+    # every small even n (all residues mod 30, partners across byte borrows),
+    # random n up to 1e8 and the edges of a 1e8 bitmap, in one launch
+    import random
+    rng = random.Random(5)
+    ns = list(range(4, 2000, 2)) + [2 * rng.randrange(2, 5 * 10**7) for _ in range(40)] + [10**8, 10**8 - 2]
+    assert voss.stats.goldbach_counts(ns) == oracle.goldbach_counts(ns)
+
+
 def test_sophie_germain(voss, oracle):
     for N, want in [(10, 3), (30, 6), (1000, 37), (10**6, 7746)]:
         with voss.primes.Context(N) as c:

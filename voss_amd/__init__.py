@@ -8,11 +8,11 @@ Miller-Rabin) and `serve` (continuous-batching request loop) have no
 reference counterpart.
 """
 
-from . import primes, conjectures, numbers, stats
+from . import primes, conjectures, numbers, stats, serve
 from .primes import Context, Statistics, ChebyshevBias, LargeGap
 from .exceptions import VossError
 
 __version__ = "0.7.0"
 
 __all__ = ["primes", "conjectures", "numbers", "stats", "Context", "Statistics",
-           "ChebyshevBias", "LargeGap", "VossError"]
+           "ChebyshevBias", "LargeGap", "VossError", "serve"]

@@ -63,6 +63,29 @@ class AmdRange(ctypes.Structure):
                 ("hist", ctypes.c_uint32 * 257), ("pad", ctypes.c_uint32)]
 
 
+class AmdRequest(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("n", ctypes.c_uint32), ("a", ctypes.c_uint64),
+                ("b", ctypes.c_uint64), ("x", ctypes.c_uint64 * 64)]
+
+
+class AmdResponse(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("pad", ctypes.c_uint32), ("value", ctypes.c_uint64),
+                ("first", ctypes.c_uint64), ("last", ctypes.c_uint64), ("hist", ctypes.c_uint32 * 257),
+                ("pad2", ctypes.c_uint32), ("latency_ms", ctypes.c_double), ("tick", ctypes.c_uint64)]
+
+
+class AmdServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int32), ("max_batch", ctypes.c_uint32), ("hbm_reserve", ctypes.c_uint64),
+                ("hbm_budget", ctypes.c_uint64), ("nth_max", ctypes.c_uint64), ("goldbach_max", ctypes.c_uint64),
+                ("plain", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class AmdServeStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("wall_s", "qps", "p50_ms", "p99_ms", "mean_ms", "max_ms",
+                                               "warmup_s")] + \
+               [(n, ctypes.c_uint64) for n in ("ticks", "max_tick", "deferred_by_memory", "peak_inflight_bytes")]
+
+
 PROGRESS_CB = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                ctypes.c_uint64, ctypes.c_void_p)
 
@@ -130,6 +153,12 @@ _SIGNATURES = {
     "voss_amd_last_kernel_ms": ([_P(ctypes.c_double), _P(ctypes.c_double), _P(ctypes.c_double), _pu64], _int),
     "voss_amd_device_info": ([_pint, ctypes.c_char_p, _int], _int),
     "voss_amd_synchronize": ([], _int),
+    "voss_amd_server_default_config": ([_P(AmdServerConfig)], _int),
+    "voss_amd_server_start": ([_P(AmdServerConfig)], _int),
+    "voss_amd_server_stop": ([], _int),
+    "voss_amd_server_submit": ([_P(AmdRequest), _pu64], _int),
+    "voss_amd_server_wait": ([_u64, _P(AmdResponse)], _int),
+    "voss_amd_server_replay": ([_vp, _vp, _u64, _vp, _P(AmdServeStats)], _int),
 }
 
 _lib = ctypes.CDLL(_find_library())

@@ -16,6 +16,7 @@
 
 #include "engine.h"
 #include "host_math.h"
+#include "serve.h"
 #include "voss/voss.h"
 #include "voss/voss_amd.h"
 #include "voss/voss_numbers.h"
@@ -723,6 +724,162 @@ extern "C" int voss_amd_device_info(int* num_cus, char* arch, int arch_len) {
 extern "C" int voss_amd_synchronize(void) {
   return guarded([&] {
     voss::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return VOSS_OK;
+  });
+}
+
+// ====================================================================== serving
+namespace {
+std::mutex g_srv_mu;
+std::unique_ptr<voss::Server> g_srv;
+
+voss::ServeConfig to_serve_config(const voss_amd_server_config_t* c) {
+  voss::ServeConfig s;
+  if (!c) return s;
+  s.slots = c->slots;
+  s.max_batch = c->max_batch;
+  s.hbm_reserve = c->hbm_reserve;
+  s.hbm_budget = c->hbm_budget;
+  s.nth_max = c->nth_max;
+  s.goldbach_max = c->goldbach_max;
+  s.plain = c->plain != 0;
+  return s;
+}
+
+void to_request(const voss_amd_request_t& in, voss::Request& r) {
+  r.kind = in.kind;
+  r.nx = in.n;
+  r.a = in.a;
+  r.b = in.b;
+  std::memcpy(r.x, in.x, sizeof(r.x));
+}
+
+void to_response(const voss::Response& r, voss_amd_response_t* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->status = r.status;
+  o->value = r.value;
+  o->first = r.first;
+  o->last = r.last;
+  std::memcpy(o->hist, r.hist, sizeof(o->hist));
+  o->latency_ms = r.latency_ms;
+  o->tick = r.batch;
+}
+}  // namespace
+
+extern "C" int voss_amd_server_default_config(voss_amd_server_config_t* cfg) {
+  if (!cfg) { set_err("cfg pointer is null"); return VOSS_ERR_INVALID_ARG; }
+  const voss::ServeConfig d;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->slots = d.slots;
+  cfg->max_batch = d.max_batch;
+  cfg->hbm_reserve = d.hbm_reserve;
+  cfg->hbm_budget = d.hbm_budget;
+  cfg->nth_max = d.nth_max;
+  cfg->goldbach_max = d.goldbach_max;
+  cfg->plain = d.plain;
+  return VOSS_OK;
+}
+
+extern "C" int voss_amd_server_start(const voss_amd_server_config_t* cfg) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    g_srv.reset();
+    g_srv.reset(new voss::Server(to_serve_config(cfg)));
+    return VOSS_OK;
+  });
+}
+
+extern "C" int voss_amd_server_stop(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    g_srv.reset();
+    return VOSS_OK;
+  });
+}
+
+extern "C" int voss_amd_server_submit(const voss_amd_request_t* req, uint64_t* ticket) {
+  if (!req || !ticket) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    if (!g_srv) throw Error(VOSS_ERR_INVALID_ARG, "server not started");
+    voss::Request r;
+    to_request(*req, r);
+    *ticket = g_srv->submit(r, voss::Server::Clock::now());
+    return VOSS_OK;
+  });
+}
+
+extern "C" int voss_amd_server_wait(uint64_t ticket, voss_amd_response_t* out) {
+  if (!out) { set_err("out pointer is null"); return VOSS_ERR_INVALID_ARG; }
+  return guarded([&] {
+    voss::Server* s = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_srv_mu);
+      s = g_srv.get();
+    }
+    if (!s) throw Error(VOSS_ERR_INVALID_ARG, "server not started");
+    to_response(s->wait(ticket), out);
+    return VOSS_OK;
+  });
+}
+
+extern "C" int voss_amd_server_replay(const voss_amd_request_t* reqs, const double* arrival_s, uint64_t n,
+                                      voss_amd_response_t* out, voss_amd_serve_stats_t* stats) {
+  if ((!reqs || !arrival_s || !out) && n) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_srv_mu);
+    if (!g_srv) throw Error(VOSS_ERR_INVALID_ARG, "server not started");
+    voss::Server& S = *g_srv;
+    using Clock = voss::Server::Clock;
+    const voss::ServeStats before = S.stats();
+    std::vector<uint64_t> tickets(n);
+    std::vector<Clock::time_point> when(n);
+    const Clock::time_point t0 = Clock::now() + std::chrono::milliseconds(2);
+    for (uint64_t i = 0; i < n; ++i) {
+      when[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(arrival_s[i]));
+      // coarse sleep, then spin the last 200 us: arrivals within ~10 us of schedule
+      const auto early = when[i] - std::chrono::microseconds(200);
+      if (Clock::now() < early) std::this_thread::sleep_until(early);
+      while (Clock::now() < when[i]) {
+      }
+      voss::Request r;
+      to_request(reqs[i], r);
+      tickets[i] = S.submit(r, when[i]);
+    }
+    std::vector<double> lat(n);
+    double end_s = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+      voss::Response r = S.wait(tickets[i]);
+      to_response(r, out + i);
+      lat[i] = r.latency_ms;
+      end_s = std::max(end_s, std::chrono::duration<double>(when[i] - t0).count() + r.latency_ms / 1e3);
+    }
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      const voss::ServeStats after = S.stats();
+      stats->wall_s = end_s - (n ? std::chrono::duration<double>(when[0] - t0).count() : 0.0);
+      stats->qps = stats->wall_s > 0 ? (double)n / stats->wall_s : 0;
+      if (n) {
+        std::vector<double> sl = lat;
+        std::sort(sl.begin(), sl.end());
+        auto pct = [&](double q) {  // nearest rank
+          size_t k = (size_t)std::ceil(q * (double)n);
+          if (k < 1) k = 1;
+          return sl[k - 1];
+        };
+        stats->p50_ms = pct(0.50);
+        stats->p99_ms = pct(0.99);
+        stats->max_ms = sl.back();
+        double sum = 0;
+        for (double v : sl) sum += v;
+        stats->mean_ms = sum / (double)n;
+      }
+      stats->warmup_s = after.warmup_s;
+      stats->ticks = after.ticks - before.ticks;
+      stats->max_tick = after.max_tick;
+      stats->deferred_by_memory = after.deferred_by_memory - before.deferred_by_memory;
+      stats->peak_inflight_bytes = after.peak_inflight_bytes;
+    }
     return VOSS_OK;
   });
 }

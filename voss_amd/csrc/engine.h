@@ -131,6 +131,25 @@ class Engine {
 
   KernelTimes last_times() const { return times_; }
 
+  // ---- batched ranges split into host layout, staging and read-back, so a
+  // caller with its own stream (the request server) can run them
+  // asynchronously.  All three need only the resident tables.
+  struct BatchLayout {
+    std::vector<uint32_t> order;  // caller index per device query, costliest first
+    uint32_t nq = 0, nseg = 0;
+  };
+  BatchLayout layout_batch(const uint64_t* a, const uint64_t* b, size_t n, uint32_t seg_bytes) const;
+  // capture a batch pipeline for up to qcap queries / scap segments on stream st
+  std::unique_ptr<BatchPlan> make_batch_plan(uint32_t qcap, uint32_t scap, const TuneConfig& c, hipStream_t st);
+  static void stage_batch(BatchPlan& P, const BatchLayout& L, const uint64_t* a, const uint64_t* b);
+  // results (including the seeds 2, 3, 5) into out[0..n); P may be null when L.nq == 0
+  static void read_batch(const BatchPlan* P, const BatchLayout& L, const uint64_t* a, const uint64_t* b, size_t n,
+                         RangeOut* out);
+  SieveArgs base_args(uint32_t nseg, const TuneConfig& c) const;
+  int grid_for(uint32_t nseg, const TuneConfig& c) const;
+  // per-segment prime counts of [7, hi] (segments of c.seg_bytes from k = 0)
+  uint32_t segment_counts(uint64_t hi, std::vector<uint32_t>& counts, const TuneConfig& c);
+
   // exposed for tests and tuning
   void ensure_tables(uint64_t max_value);
   uint32_t tier_index(uint32_t p) const;  // #sieving primes < p
@@ -140,12 +159,10 @@ class Engine {
  private:
   Engine();
   ~Engine();
-  SieveArgs base_args(uint32_t nseg) const;
-  int grid_for(uint32_t nseg) const;
+  SieveArgs base_args(uint32_t nseg) const { return base_args(nseg, cfg_); }
+  int grid_for(uint32_t nseg) const { return grid_for(nseg, cfg_); }
   void load_tune_file();
   uint64_t count_bitmap(uint64_t lo, uint64_t hi);  // wheel-bitmap primes in [max(lo,7), hi]
-  // per-segment prime counts of [7, hi] into summaries (count field), returns #segments
-  uint32_t segment_counts(uint64_t hi, std::vector<uint32_t>& counts);
 
   BatchPlan& batch_plan(uint32_t nq, uint32_t nseg);
   std::vector<std::unique_ptr<BatchPlan>> plans_;

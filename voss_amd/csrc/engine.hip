@@ -25,7 +25,7 @@ namespace voss {
 
 namespace {
 const uint32_t kW30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
-constexpr int kErrNoDevice = 3, kErrOom = 4, kErrDevice = 5, kErrArg = 6;
+constexpr int kErrNoDevice = 3, kErrOom = 4, kErrDevice = 5, kErrArg = 6, kErrInternal = 7;
 
 uint64_t isqrt64(uint64_t n) {
   uint64_t r = (uint64_t)std::sqrt((double)n);
@@ -148,7 +148,9 @@ Engine::Engine() {
       if (pr) first_word_ |= 1u << (8 * byte + o);
     }
   hip_check(prepare_kernels(lds_bytes_for(kMaxSegBytes)), "prepare_kernels");
-  ensure_tables(1000000000000ull);
+  // sieving primes up to 2^24 cover every value below 2.8e14; built once so
+  // concurrent users (the request server's slots) never see them move
+  ensure_tables(1ull << 48);
 }
 
 Engine::~Engine() {}
@@ -204,6 +206,9 @@ void Engine::set_config(const TuneConfig& c, int kind) {
 void Engine::ensure_tables(uint64_t max_value) {
   const uint64_t need = isqrt64(max_value) + 2;
   if (need <= table_limit_) return;
+  // the table is built once, at its largest, in the constructor: it never
+  // moves under a caller that holds its address
+  if (table_limit_) throw Error(kErrArg, "value beyond the sieving-prime table (2.8e14)");
   uint64_t lim = std::max<uint64_t>(need + need / 4, 1u << 16);
   if (lim > (1ull << 24) - 1) lim = (1ull << 24) - 1;  // 1.67e7 > sqrt(1e14)
   std::vector<uint8_t> s(lim + 1, 1);
@@ -243,7 +248,7 @@ uint32_t Engine::tier_index(uint32_t p) const {
   return (uint32_t)(std::lower_bound(h_primes_.begin(), h_primes_.end(), p) - h_primes_.begin());
 }
 
-SieveArgs Engine::base_args(uint32_t nseg) const {
+SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   SieveArgs A;
   std::memset(&A, 0, sizeof(A));
   A.primes = d_primes_;
@@ -252,16 +257,16 @@ SieveArgs Engine::base_args(uint32_t nseg) const {
   A.presieve = d_presieve_;
   for (int k = 0; k < kPresieveGroups; ++k) A.presieve_off[k] = presieve_off_[k];
   A.first_word = first_word_;
-  A.seg_bytes = cfg_.seg_bytes;
+  A.seg_bytes = c.seg_bytes;
   A.nseg = nseg;
-  A.tier1_end = tier_index(cfg_.tier1_p);
-  A.tier2_end = tier_index(cfg_.seg_bytes / 4);
+  A.tier1_end = tier_index(c.tier1_p);
+  A.tier2_end = tier_index(c.seg_bytes / 4);
   A.large_threshold = 0xFFFFFFFFu;
   return A;
 }
 
-int Engine::grid_for(uint32_t nseg) const {
-  uint64_t g = (uint64_t)ncu_ * (uint64_t)cfg_.wgs_per_cu;
+int Engine::grid_for(uint32_t nseg, const TuneConfig& c) const {
+  uint64_t g = (uint64_t)ncu_ * (uint64_t)c.wgs_per_cu;
   if (g > nseg) g = nseg;
   return (int)std::max<uint64_t>(g, 1);
 }
@@ -551,21 +556,16 @@ static uint32_t pow2_at_least(uint64_t v, uint32_t lo) {
   return (uint32_t)c;
 }
 
-BatchPlan& Engine::batch_plan(uint32_t nq, uint32_t nseg) {
-  const uint32_t qcap = pow2_at_least(nq, 64), scap = pow2_at_least(nseg, 1024);
-  for (auto& p : plans_)
-    if (p->qcap == qcap && p->scap == scap && p->seg_bytes == cfg_.seg_bytes && p->block == cfg_.block &&
-        p->wgs_per_cu == cfg_.wgs_per_cu && p->tier1_p == cfg_.tier1_p && p->table_limit == table_limit_)
-      return *p;
-  if (plans_.size() >= 8) plans_.erase(plans_.begin());
-  plans_.emplace_back(new BatchPlan());
-  BatchPlan& P = *plans_.back();
+std::unique_ptr<BatchPlan> Engine::make_batch_plan(uint32_t qcap, uint32_t scap, const TuneConfig& c,
+                                                   hipStream_t st) {
+  std::unique_ptr<BatchPlan> up(new BatchPlan());
+  BatchPlan& P = *up;
   P.qcap = qcap;
   P.scap = scap;
-  P.seg_bytes = cfg_.seg_bytes;
-  P.block = cfg_.block;
-  P.wgs_per_cu = cfg_.wgs_per_cu;
-  P.tier1_p = cfg_.tier1_p;
+  P.seg_bytes = c.seg_bytes;
+  P.block = c.block;
+  P.wgs_per_cu = c.wgs_per_cu;
+  P.tier1_p = c.tier1_p;
   P.table_limit = table_limit_;
   P.in_bytes = 64 + (size_t)qcap * sizeof(QueryDesc) + (size_t)scap * 4;
   P.out_bytes = (size_t)qcap * 24 + (size_t)qcap * kQueryBins * 4;
@@ -582,7 +582,7 @@ BatchPlan& Engine::batch_plan(uint32_t nq, uint32_t nseg) {
   unsigned long long* d_qc = P.d_out.as<unsigned long long>();
   unsigned int* d_qh = reinterpret_cast<unsigned int*>(P.d_out.as<uint8_t>() + (size_t)qcap * 24);
 
-  SieveArgs A = base_args(scap);
+  SieveArgs A = base_args(scap, c);
   A.work = P.d_work.as<unsigned int>();
   A.nseg_dev = d_hdr + 1;
   A.summaries = P.d_sum.as<SegSummary>();
@@ -595,83 +595,89 @@ BatchPlan& Engine::batch_plan(uint32_t nq, uint32_t nseg) {
   B.summaries = P.d_sum.as<SegSummary>();
   B.nq = qcap;
   B.nq_dev = d_hdr;
-  B.seg_bytes = cfg_.seg_bytes;
+  B.seg_bytes = c.seg_bytes;
   B.qhist = d_qh;
   B.first = d_qc + qcap;
   B.last = d_qc + 2 * qcap;
 
-  hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-  hip_check(hipMemcpyAsync(P.d_in.p, P.h_in, P.in_bytes, hipMemcpyHostToDevice, stream_), "capture H2D");
-  hip_check(launch_zero(P.d_out.as<uint32_t>(), P.out_bytes / 4, stream_), "capture zero");
-  hip_check(launch_zero(P.d_work.as<uint32_t>(), 64, stream_), "capture zero");
-  hip_check(launch_sieve(kModeBatch, cfg_.block, A, grid_for(scap), stream_), "capture sieve");
-  hip_check(launch_batch_stitch(B, qcap, stream_), "capture stitch");
-  hip_check(hipMemcpyAsync(P.h_out, P.d_out.p, P.out_bytes, hipMemcpyDeviceToHost, stream_), "capture D2H");
-  hip_check(hipStreamEndCapture(stream_, &P.graph), "hipStreamEndCapture");
+  hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hip_check(hipMemcpyAsync(P.d_in.p, P.h_in, P.in_bytes, hipMemcpyHostToDevice, st), "capture H2D");
+  hip_check(launch_zero(P.d_out.as<uint32_t>(), P.out_bytes / 4, st), "capture zero");
+  hip_check(launch_zero(P.d_work.as<uint32_t>(), 64, st), "capture zero");
+  hip_check(launch_sieve(kModeBatch, c.block, A, grid_for(scap, c), st), "capture sieve");
+  hip_check(launch_batch_stitch(B, qcap, st), "capture stitch");
+  hip_check(hipMemcpyAsync(P.h_out, P.d_out.p, P.out_bytes, hipMemcpyDeviceToHost, st), "capture D2H");
+  hip_check(hipStreamEndCapture(st, &P.graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-  return P;
+  return up;
 }
 
-void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeOut* out) {
-  cfg_ = cfgs_[1];
-  const uint32_t S = cfg_.seg_bytes;
-  std::vector<uint32_t> order;  // caller index, costliest segments first
-  uint64_t bmax = 7;
+BatchPlan& Engine::batch_plan(uint32_t nq, uint32_t nseg) {
+  const uint32_t qcap = pow2_at_least(nq, 64), scap = pow2_at_least(nseg, 1024);
+  for (auto& p : plans_)
+    if (p->qcap == qcap && p->scap == scap && p->seg_bytes == cfg_.seg_bytes && p->block == cfg_.block &&
+        p->wgs_per_cu == cfg_.wgs_per_cu && p->tier1_p == cfg_.tier1_p && p->table_limit == table_limit_)
+      return *p;
+  if (plans_.size() >= 8) plans_.erase(plans_.begin());
+  plans_.push_back(make_batch_plan(qcap, scap, cfg_, stream_));
+  return *plans_.back();
+}
+
+Engine::BatchLayout Engine::layout_batch(const uint64_t* a, const uint64_t* b, size_t n, uint32_t S) const {
+  BatchLayout L;
   for (size_t i = 0; i < n; ++i) {
-    out[i] = RangeOut();
     const uint64_t lo = std::max<uint64_t>(a[i], 7);
     if (b[i] < lo) continue;
-    order.push_back((uint32_t)i);
-    bmax = std::max(bmax, b[i]);
+    L.order.push_back((uint32_t)i);
   }
   // per-segment cost grows with sqrt(b): deal the expensive queries first
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return b[x] > b[y]; });
-  const uint32_t nq = (uint32_t)order.size();
-  times_ = KernelTimes();
-  if (nq) {
-    ensure_tables(bmax);
-    uint64_t total = 0;
-    for (uint32_t q = 0; q < nq; ++q) {
-      const uint64_t lo = std::max<uint64_t>(a[order[q]], 7);
-      total += ((b[order[q]] / 30 + 1) - lo / 30 + S - 1) / S;
-    }
-    if (total > 0x40000000ull) throw Error(kErrArg, "batch too large");
-    const uint32_t nseg = (uint32_t)total;
-    BatchPlan& P = batch_plan(nq, nseg);
-    uint8_t* hin = static_cast<uint8_t*>(P.h_in);
-    uint32_t* hdr = reinterpret_cast<uint32_t*>(hin);
-    QueryDesc* hq = reinterpret_cast<QueryDesc*>(hin + 64);
-    uint32_t* hsq = reinterpret_cast<uint32_t*>(hin + 64 + (size_t)P.qcap * sizeof(QueryDesc));
-    hdr[0] = nq;
-    hdr[1] = nseg;
-    uint32_t seg = 0;
-    for (uint32_t q = 0; q < nq; ++q) {
-      const uint32_t i = order[q];
-      QueryDesc& Q = hq[q];
-      Q.a = std::max<uint64_t>(a[i], 7);
-      Q.b = b[i];
-      Q.k_lo = Q.a / 30;
-      Q.k_hi = b[i] / 30 + 1;
-      Q.seg_first = seg;
-      Q.nseg = (uint32_t)((Q.k_hi - Q.k_lo + S - 1) / S);
-      for (uint32_t k = 0; k < Q.nseg; ++k) hsq[seg + k] = q;
-      seg += Q.nseg;
-    }
-    Event e0, e1;
-    hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
-    hip_check(hipGraphLaunch(P.exec, stream_), "hipGraphLaunch");
-    hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
-    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    times_.sieve_ms = elapsed(e0, e1);
-    times_.total_ms = times_.sieve_ms;
-    times_.segments = nseg;
-    const unsigned long long* qc = static_cast<const unsigned long long*>(P.h_out);
-    const uint32_t* qh = reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(P.h_out) + (size_t)P.qcap * 24);
-    for (uint32_t q = 0; q < nq; ++q) {
-      RangeOut& o = out[order[q]];
+  std::stable_sort(L.order.begin(), L.order.end(), [&](uint32_t x, uint32_t y) { return b[x] > b[y]; });
+  L.nq = (uint32_t)L.order.size();
+  uint64_t total = 0;
+  for (uint32_t q = 0; q < L.nq; ++q) {
+    const uint64_t lo = std::max<uint64_t>(a[L.order[q]], 7);
+    total += ((b[L.order[q]] / 30 + 1) - lo / 30 + S - 1) / S;
+  }
+  if (total > 0x40000000ull) throw Error(kErrArg, "batch too large");
+  L.nseg = (uint32_t)total;
+  return L;
+}
+
+void Engine::stage_batch(BatchPlan& P, const BatchLayout& L, const uint64_t* a, const uint64_t* b) {
+  if (L.nq > P.qcap || L.nseg > P.scap) throw Error(kErrInternal, "batch exceeds its plan");
+  const uint32_t S = P.seg_bytes;
+  uint8_t* hin = static_cast<uint8_t*>(P.h_in);
+  uint32_t* hdr = reinterpret_cast<uint32_t*>(hin);
+  QueryDesc* hq = reinterpret_cast<QueryDesc*>(hin + 64);
+  uint32_t* hsq = reinterpret_cast<uint32_t*>(hin + 64 + (size_t)P.qcap * sizeof(QueryDesc));
+  hdr[0] = L.nq;
+  hdr[1] = L.nseg;
+  uint32_t seg = 0;
+  for (uint32_t q = 0; q < L.nq; ++q) {
+    const uint32_t i = L.order[q];
+    QueryDesc& Q = hq[q];
+    Q.a = std::max<uint64_t>(a[i], 7);
+    Q.b = b[i];
+    Q.k_lo = Q.a / 30;
+    Q.k_hi = b[i] / 30 + 1;
+    Q.seg_first = seg;
+    Q.nseg = (uint32_t)((Q.k_hi - Q.k_lo + S - 1) / S);
+    for (uint32_t k = 0; k < Q.nseg; ++k) hsq[seg + k] = q;
+    seg += Q.nseg;
+  }
+}
+
+void Engine::read_batch(const BatchPlan* P, const BatchLayout& L, const uint64_t* a, const uint64_t* b, size_t n,
+                        RangeOut* out) {
+  for (size_t i = 0; i < n; ++i) out[i] = RangeOut();
+  if (L.nq) {
+    const unsigned long long* qc = static_cast<const unsigned long long*>(P->h_out);
+    const uint32_t* qh = reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(P->h_out) + (size_t)P->qcap * 24);
+    for (uint32_t q = 0; q < L.nq; ++q) {
+      RangeOut& o = out[L.order[q]];
       o.count = qc[q];
-      o.first = qc[P.qcap + q];
-      o.last = qc[2 * P.qcap + q];
+      o.first = qc[P->qcap + q];
+      o.last = qc[2 * P->qcap + q];
       std::memcpy(o.hist, qh + (size_t)q * kQueryBins, kQueryBins * 4);
     }
   }
@@ -679,17 +685,42 @@ void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeO
   for (size_t i = 0; i < n; ++i) {
     if (a[i] > 5 || b[i] < 2 || b[i] < a[i]) continue;
     RangeOut& o = out[i];
-    std::vector<uint64_t> sd;
-    for (uint64_t s : {2ull, 3ull, 5ull}) if (s >= a[i] && s <= b[i]) sd.push_back(s);
-    if (sd.empty()) continue;
+    uint64_t sd[3];
+    int ns = 0;
+    for (uint64_t s : {2ull, 3ull, 5ull}) if (s >= a[i] && s <= b[i]) sd[ns++] = s;
+    if (!ns) continue;
     auto add_gap = [&](uint64_t g) {
       if (g >= 2 && g <= 512 && !(g & 1)) o.hist[g / 2]++; else o.hist[0]++;
     };
-    for (size_t k = 1; k < sd.size(); ++k) add_gap(sd[k] - sd[k - 1]);
-    if (o.count) add_gap(o.first - sd.back());
-    else o.last = sd.back();
-    o.first = sd.front();
-    o.count += sd.size();
+    for (int k = 1; k < ns; ++k) add_gap(sd[k] - sd[k - 1]);
+    if (o.count) add_gap(o.first - sd[ns - 1]);
+    else o.last = sd[ns - 1];
+    o.first = sd[0];
+    o.count += ns;
+  }
+}
+
+void Engine::batch_ranges(const uint64_t* a, const uint64_t* b, size_t n, RangeOut* out) {
+  cfg_ = cfgs_[1];
+  BatchLayout L = layout_batch(a, b, n, cfg_.seg_bytes);
+  times_ = KernelTimes();
+  if (L.nq) {
+    uint64_t bmax = 7;
+    for (uint32_t i : L.order) bmax = std::max(bmax, b[i]);
+    ensure_tables(bmax);
+    BatchPlan& P = batch_plan(L.nq, L.nseg);
+    stage_batch(P, L, a, b);
+    Event e0, e1;
+    hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+    hip_check(hipGraphLaunch(P.exec, stream_), "hipGraphLaunch");
+    hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    times_.sieve_ms = elapsed(e0, e1);
+    times_.total_ms = times_.sieve_ms;
+    times_.segments = L.nseg;
+    read_batch(&P, L, a, b, n, out);
+  } else {
+    read_batch(nullptr, L, a, b, n, out);
   }
 }
 

@@ -73,8 +73,10 @@ std::vector<uint64_t> Engine::goldbach_counts(const uint64_t* ns, size_t k) {
   bitmap_range(7, nmax, bm, &k_base);
   DevBuf d_ns(k * 8), d_out(k * 8);
   hip_check(hipMemcpyAsync(d_ns.p, ns, k * 8, hipMemcpyHostToDevice, stream_), "H2D");
-  hip_check(launch_goldbach_count(bm.as<uint8_t>(), d_ns.as<uint64_t>(), k, d_out.as<unsigned long long>(), stream_),
-            "launch(goldbach_count)");
+  for (size_t i0 = 0; i0 < k; i0 += 65535)
+    hip_check(launch_goldbach_count(bm.as<uint8_t>(), d_ns.as<uint64_t>() + i0, std::min<size_t>(65535, k - i0), nmax,
+                                    d_out.as<unsigned long long>() + i0, stream_),
+              "launch(goldbach_count)");
   hip_check(hipMemcpyAsync(out.data(), d_out.p, k * 8, hipMemcpyDeviceToHost, stream_), "D2H");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   return out;
@@ -116,8 +118,8 @@ void Engine::next_prev_batch(const uint64_t* x, size_t n, uint64_t* out, int dir
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
-uint32_t Engine::segment_counts(uint64_t hi, std::vector<uint32_t>& counts) {
-  cfg_ = cfgs_[0];
+uint32_t Engine::segment_counts(uint64_t hi, std::vector<uint32_t>& counts, const TuneConfig& c) {
+  cfg_ = c;
   ensure_tables(hi);
   const uint32_t S = cfg_.seg_bytes;
   const uint64_t k_end = hi / 30 + 1;
@@ -155,7 +157,7 @@ uint64_t Engine::nth_prime(uint64_t n) {
   // Rosser: p_n < n (ln n + ln ln n) for n >= 6
   const uint64_t hi = (uint64_t)(nf * (std::log(nf) + std::log(std::log(nf)))) + 64;
   std::vector<uint32_t> counts;
-  const uint32_t nseg = segment_counts(hi, counts);
+  const uint32_t nseg = segment_counts(hi, counts, cfgs_[0]);
   uint64_t cum = 3;  // 2, 3, 5
   uint32_t s = 0;
   for (; s < nseg; ++s) {

@@ -55,6 +55,10 @@ hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipSt
 hipError_t launch_stitch(const StitchArgs& A, hipStream_t st);
 hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st);
 hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStream_t st);
+// k-th prime (1-based ranks[i]) of sweep segment segs[i] for i < *n_dev; one
+// workgroup per request, n_cap workgroups launched.
+hipError_t launch_select(int block, const SieveArgs& A, const uint32_t* segs, const uint32_t* ranks,
+                         unsigned long long* out, const uint32_t* n_dev, uint32_t n_cap, hipStream_t st);
 // Zero n 32-bit words (graph-safe fill, no memset node).
 hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st);
 

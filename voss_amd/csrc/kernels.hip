@@ -999,6 +999,107 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   }
 }
 
+// ---------------------------------------------------------------- select
+// One workgroup per request: sieve sweep segment segs[i] (byte k0 = seg *
+// seg_bytes, values >= 7 only) and return its ranks[i]-th prime (1-based),
+// or 0 when the segment holds fewer.
+template <int BS>
+__global__ void __launch_bounds__(BS) select_kernel(SieveArgs A, const uint32_t* segs, const uint32_t* ranks,
+                                                    unsigned long long* out, const uint32_t* n_dev) {
+  extern __shared__ uint32_t lds[];
+  constexpr int NW = BS / 64;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (blockIdx.x >= *n_dev) return;  // uniform per workgroup
+  init_tables(lds);
+  __syncthreads();
+  const uint32_t s = segs[blockIdx.x];
+  const uint32_t rank = ranks[blockIdx.x];
+  SegGeom g;
+  g.query = 0;
+  g.k0 = (uint64_t)s * A.seg_bytes;
+  const uint64_t rem = A.k_end - g.k0;
+  g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
+  g.lo_val = 7;
+  g.hi_val = A.hi_val;
+  g.vlo = bit_at_or_above(g.lo_val, g.k0, g.nbytes);
+  g.vhi = bit_at_or_above(g.hi_val + 1, g.k0, g.nbytes);
+  init_segment<BS>(A, lds, g);
+  __syncthreads();
+  sieve_segment<BS>(A, lds, g);
+  __syncthreads();
+  const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+  const uint32_t wave_bytes = A.seg_bytes / NW;
+  const uint32_t wbeg = wave * wave_bytes, wend = min(wbeg + wave_bytes, g.nbytes);
+  uint32_t* wcount = lds + kLdsMisc + 16;  // NW words
+  uint32_t c = 0;
+  for (uint32_t base = wbeg; base < wend; base += 1024) {
+    uint4 q = *reinterpret_cast<const uint4*>(seg8 + base + 16 * lane);
+    const uint32_t cb = 8 * (base + 16 * lane);
+    if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+    c += popc4(q);
+  }
+  c = wave_sum(c);
+  if (lane == 0) wcount[wave] = c;
+  __syncthreads();
+  // the wave holding the rank-th prime, and the rank within it
+  uint32_t before = 0, target_wave = NW;
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t cw = wcount[w];
+    if (target_wave == NW && before + cw >= rank) target_wave = w;
+    else if (target_wave == NW) before += cw;
+  }
+  if (target_wave == NW || rank == 0) {
+    if (threadIdx.x == 0) out[blockIdx.x] = 0;
+    return;
+  }
+  if (wave != target_wave) return;
+  uint32_t need = rank - before;  // 1-based within this wave's region
+  for (uint32_t base = wbeg; base < wend; base += 1024) {
+    uint4 q = *reinterpret_cast<const uint4*>(seg8 + base + 16 * lane);
+    const uint32_t boff = base + 16 * lane;
+    const uint32_t cb = 8 * boff;
+    if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+    const uint32_t pc = popc4(q);
+    uint32_t incl = pc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+      if (lane >= (uint32_t)d) incl += t;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+    if (need <= total) {
+      const uint32_t excl = incl - pc;
+      if (need > excl && need <= incl) {
+        uint32_t k = need - excl;  // k-th set bit of this lane's chunk
+        const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+        for (int wi = 0; wi < 4; ++wi) {
+          uint32_t w = ws[wi];
+          const uint32_t pw = __builtin_popcount(w);
+          if (k > pw) { k -= pw; continue; }
+          for (uint32_t t = 1; t < k; ++t) w &= w - 1;
+          const uint32_t b = __builtin_ctz(w);
+          out[blockIdx.x] = 30ull * (g.k0 + boff + 4 * wi + (b >> 3)) + w30(b & 7);
+          break;
+        }
+      }
+      break;  // wave-uniform
+    }
+    need -= total;
+  }
+}
+
+hipError_t launch_select(int block, const SieveArgs& A, const uint32_t* segs, const uint32_t* ranks,
+                         unsigned long long* out, const uint32_t* n_dev, uint32_t n_cap, hipStream_t st) {
+  if (!n_cap) return hipSuccess;
+  if (block == 1024)
+    hipLaunchKernelGGL((select_kernel<1024>), dim3(n_cap), dim3(1024), lds_bytes_for(A.seg_bytes), st, A, segs, ranks,
+                       out, n_dev);
+  else
+    hipLaunchKernelGGL((select_kernel<512>), dim3(n_cap), dim3(512), lds_bytes_for(A.seg_bytes), st, A, segs, ranks,
+                       out, n_dev);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- batch stitch
 // One thread per query: join its segments in order (boundary gaps into the
 // query histogram) and report first and last prime.
@@ -1114,7 +1215,8 @@ hipError_t prepare_kernels(uint32_t lds_bytes) {
       (const void*)sieve_kernel<kModeBatch, 1024>, (const void*)sieve_kernel<kModeCount, 512>,
       (const void*)sieve_kernel<kModeStats, 512>,  (const void*)sieve_kernel<kModeBatch, 512>,
       (const void*)sieve_kernel<kModeBitmap, 1024>, (const void*)sieve_kernel<kModeBitmap, 512>,
-      (const void*)rescan_kernel<1024>,            (const void*)rescan_kernel<512>};
+      (const void*)rescan_kernel<1024>,            (const void*)rescan_kernel<512>,
+      (const void*)select_kernel<1024>,            (const void*)select_kernel<512>};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;

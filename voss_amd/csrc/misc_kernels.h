@@ -11,9 +11,10 @@ hipError_t launch_next_prev(const uint64_t* x, uint64_t n, uint64_t* out, int di
 // bm: wheel bitmap from byte 0 covering [0, N]; sp: ascending odd primes 3, 5, 7, ...
 hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t N, uint64_t* ce,
                                  unsigned long long* nce, uint64_t max_ce, hipStream_t st);
-// bm covers [0, max(ns)]; one workgroup per n.
-hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, unsigned long long* out,
-                                 hipStream_t st);
+// bm covers [0, n_max] with n_max >= every ns[i]; k <= 65535.  Writes out[i]
+// (zeroed by the launch) from a grid of (chunks of n/2, requests).
+hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, uint64_t n_max,
+                                 unsigned long long* out, hipStream_t st);
 // bm covers [0, 2N+1]; adds the count of primes p in [7, N] with 2p+1 prime to *out.
 hipError_t launch_sophie(const uint8_t* bm, uint64_t N, unsigned long long* out, hipStream_t st);
 

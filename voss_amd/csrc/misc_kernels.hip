@@ -137,7 +137,8 @@ __global__ void next_prev_kernel(const uint64_t* x, uint64_t n, uint64_t* out, i
       if (c == 2) res = 2;
       else {
         if ((c & 1) == 0) ++c;
-        for (uint64_t steps = 0; steps < 100000000ull; steps += 2, c += 2)
+        // c > v stops at the top of uint64 (no prime above 2^64 - 59): res stays 0
+        for (uint64_t steps = 0; steps < 100000000ull && c > v; steps += 2, c += 2)
           if (mr_is_prime(c)) { res = c; break; }
       }
     }
@@ -176,37 +177,93 @@ __global__ void goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uin
   if (slot < max_ce) ce[slot] = n;
 }
 
-// One workgroup (256 threads) per n: count primes p <= n/2 with n - p prime.
-__global__ void __launch_bounds__(256) goldbach_count_kernel(const uint8_t* bm, const uint64_t* ns,
-                                                             unsigned long long* out) {
-  __shared__ unsigned long long red[4];
-  const uint64_t n = ns[blockIdx.x];
+// Goldbach partition counts as bit-plane correlations of the wheel bitmap.
+// For n = 30K + s and p = 30k + W[i] (W = 1, 7, ..., 29), the partner
+// q = n - p has residue d = (s - W[i]) mod 30 and byte index K - k - borrow
+// (borrow = 1 when s < W[i]).  q is a candidate only when d is a wheel
+// residue j, so the count over p in [7, n/2] is
+//   sum_i #{k : bit_i(bm[k]) and bit_j(bm[K - borrow_i - k])},
+// evaluated four low bytes at a time against the byte-reversed high side.
+// blockIdx.y = request; blockIdx.x = chunk of kGbBytesPerBlock low bytes;
+// partial counts are added into out (zeroed beforehand).
+constexpr uint32_t kGbThreads = 256, kGbIters = 16;
+constexpr uint64_t kGbBytesPerBlock = 4ull * kGbThreads * kGbIters;
+
+__global__ void __launch_bounds__(kGbThreads) goldbach_count_kernel(const uint8_t* bm, const uint64_t* ns,
+                                                                    unsigned long long* out) {
+  __shared__ unsigned long long red[kGbThreads / 64];
+  const uint64_t n = ns[blockIdx.y];
   const uint64_t half = n / 2;
+  const uint64_t kmax = half / 30;  // last low byte that can hold p <= n/2
+  const uint64_t kbeg = (uint64_t)blockIdx.x * kGbBytesPerBlock;
+  if (n < 4 || kbeg > kmax) return;  // uniform per workgroup
   unsigned long long c = 0;
-  if (threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // p in {2, 3, 5}: 2 + 2 = 4; 3 and 5 with any prime partner q >= p
     if (n == 4) c = 1;
     if (n > 4) {
       if (3 <= half && bm_prime(bm, n - 3)) ++c;
       if (5 <= half && bm_prime(bm, n - 5)) ++c;
     }
   }
+  const uint64_t K = n / 30;
+  const uint32_t sres = (uint32_t)(n - 30 * K);
+  // per residue i: partner bit j (8 = none) and borrow
+  uint32_t jj[8], bo[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t wi = w30(i);
+    const uint32_t b = sres < wi ? 1u : 0u;
+    jj[i] = c_res2bit[sres + 30 * b - wi];
+    bo[i] = b;
+  }
+  // residues i admissible in the last byte kmax: 30 kmax + W[i] <= half
+  uint32_t last_mask = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (30 * kmax + w30(i) <= half) last_mask |= 1u << i;
   if (n > 4) {
-    const uint64_t kmax = half / 30;
-    for (uint64_t k = threadIdx.x; k <= kmax; k += 256) {
-      uint32_t x = bm[k];
-      while (x) {
-        const uint32_t o = __builtin_ctz(x);
-        x &= x - 1;
-        const uint64_t p = 30 * k + w30(o);
-        if (p < 7 || p > half) continue;
-        if (bm_prime(bm, n - p)) ++c;
+    for (uint32_t it = 0; it < kGbIters; ++it) {
+      const uint64_t k0 = kbeg + 4ull * (it * kGbThreads + threadIdx.x);
+      if (k0 > kmax) break;
+      uint32_t lo = *reinterpret_cast<const uint32_t*>(bm + k0);  // k0 is a multiple of 4
+      // clear bytes past kmax and residues past n/2 in byte kmax
+#pragma unroll
+      for (uint32_t t = 0; t < 4; ++t) {
+        const uint64_t k = k0 + t;
+        const uint32_t m = k < kmax ? 0xFFu : (k == kmax ? last_mask : 0u);
+        lo &= ~(0xFFu << (8 * t)) | (m << (8 * t));
+      }
+      // high side: hb[u] = bm[K - k0 - u], u = 0..4 (all indices >= K - kmax - 4 may go below 0)
+      uint32_t hb[5];
+#pragma unroll
+      for (uint32_t u = 0; u < 5; ++u) {
+        const uint64_t off = k0 + u;
+        hb[u] = off <= K ? bm[K - off] : 0u;
+      }
+      const uint32_t h0 = hb[0] | (hb[1] << 8) | (hb[2] << 16) | (hb[3] << 24);  // byte t = bm[K - k0 - t]
+      const uint32_t h1 = hb[1] | (hb[2] << 8) | (hb[3] << 16) | (hb[4] << 24);  // byte t = bm[K - 1 - k0 - t]
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (jj[i] == 8) continue;
+        const uint32_t hi = bo[i] ? h1 : h0;
+        c += __builtin_popcount(((lo >> i) & 0x01010101u) & ((hi >> jj[i]) & 0x01010101u));
       }
     }
   }
   c = wave_sum(c);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
   __syncthreads();
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (uint32_t w = 0; w < kGbThreads / 64; ++w) t += red[w];
+    if (t) atomicAdd(out + blockIdx.y, t);
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 // Sophie Germain: bitmap primes p in [7, N] with 2p + 1 prime.
@@ -245,10 +302,14 @@ hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t
                      ce, nce, max_ce);
   return hipGetLastError();
 }
-hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, unsigned long long* out,
-                                 hipStream_t st) {
+hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, uint64_t n_max,
+                                 unsigned long long* out, hipStream_t st) {
   if (!k) return hipSuccess;
-  hipLaunchKernelGGL(goldbach_count_kernel, dim3((uint32_t)k), dim3(256), 0, st, bm, ns, out);
+  if (k > 65535) return hipErrorInvalidValue;  // grid.y limit; callers split larger sets
+  hipLaunchKernelGGL(zero_u64_kernel, dim3((uint32_t)((k + 255) / 256)), dim3(256), 0, st, out, k);
+  const uint64_t chunks = (n_max / 2 / 30) / kGbBytesPerBlock + 1;
+  hipLaunchKernelGGL(goldbach_count_kernel, dim3((uint32_t)chunks, (uint32_t)k), dim3(kGbThreads), 0, st, bm, ns,
+                     out);
   return hipGetLastError();
 }
 hipError_t launch_sophie(const uint8_t* bm, uint64_t N, unsigned long long* out, hipStream_t st) {
