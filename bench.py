@@ -14,7 +14,18 @@ the kernel's HIP-event time on the engine stream; peak = 256 CUs x 32 LDS
 dword ops/clk x 2.4 GHz.  `cpu_baseline` times the oracle (a multithreaded
 C port of the same algorithm) on a bounded sample of the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1]
+Two more sections ride on the same line:
+  batch: 4096 ranges (seed 0, a = 10^U(6,13), width 10^U(3,9)); queries/s of
+         the batched graph launch vs one range_stats call per query, both
+         checked against the committed oracle fixture (count, first, last,
+         histogram digest).
+  serve: the mixed Poisson trace (seed 1, 2000 requests/s: 50% ranges, 20%
+         nth/next/prev, 20% is_prime x64, 10% Goldbach counts); p50/p99 and
+         queries/s of the continuous-batching server vs the same kernels run
+         one request at a time, plus the saturated (all-at-once) rate of each;
+         every response compared between the two.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1] [--no-batch] [--no-serve]
 """
 
 import argparse
@@ -64,6 +75,76 @@ def cpu_baseline(sample_n: int, reps: int):
                       f"median of {reps}: {dt:.2f} s wall, pi = {s['prime_count']}"}
 
 
+FIXTURE_C2 = ROOT / "tests" / "golden" / "ranges_c2.json"
+
+
+def fnv_digests(hist):
+    """64-bit FNV-1a of each row's bins as little-endian uint64 (the fixture's digest)."""
+    import numpy as np
+    b = np.ascontiguousarray(hist.astype("<u8")).view(np.uint8).reshape(len(hist), -1)
+    d = np.full(len(hist), 0xcbf29ce484222325, dtype=np.uint64)
+    prime = np.uint64(0x100000001b3)
+    with np.errstate(over="ignore"):
+        for j in range(b.shape[1]):
+            d ^= b[:, j].astype(np.uint64)
+            d *= prime
+    return [f"{int(v):016x}" for v in d]
+
+
+def bench_batch(st, reps: int):
+    import numpy as np
+    gold = json.loads(FIXTURE_C2.read_text())["queries"]
+    a = [g[0] for g in gold]
+    b = [g[1] for g in gold]
+    st.batch_ranges(a, b, as_numpy=True)  # warm: plan capture
+    times, kms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = st.batch_ranges(a, b, as_numpy=True)
+        times.append(time.perf_counter() - t0)
+        kms.append(st.last_kernel_ms()["total_ms"])
+    dt = sorted(times)[len(times) // 2]
+    exact = (list(map(int, r["count"])) == [g[2] for g in gold] and list(map(int, r["first"])) == [g[3] for g in gold]
+             and list(map(int, r["last"])) == [g[4] for g in gold] and fnv_digests(r["hist"]) == [g[5] for g in gold])
+    t0 = time.perf_counter()
+    plain = [st.range_stats(x, y) for x, y in zip(a, b)]
+    dt_plain = time.perf_counter() - t0
+    exact_plain = all(p["count"] == g[2] and p["first"] == g[3] and p["last"] == g[4] for p, g in zip(plain, gold))
+    return {"workload": "4096 ranges, seed 0, a=10^U(6,13), width=10^U(3,9); count, gap histogram (bins 2..512), "
+                        "first/last",
+            "batched_qps": len(a) / dt, "batched_ms": dt * 1e3, "batched_device_ms": sorted(kms)[len(kms) // 2],
+            "plain_qps": len(a) / dt_plain, "plain_ms": dt_plain * 1e3,
+            "speedup": dt_plain / dt, "exact_vs_fixture": bool(exact and exact_plain),
+            "engine": st.get_config("batch")}
+
+
+def bench_serve(n: int):
+    import numpy as np
+    from voss_amd import serve as S
+    reqs, arr = S.make_trace(n, rate=2000.0, seed=1)
+    out = {"workload": f"{n} requests, Poisson 2000/s seed 1: 50% ranges (as batch), 20% nth/next/prev "
+                       "(args 10^U(0,12)), 20% is_prime x64 uint64, 10% Goldbach counts n<=1e8",
+           "plain_path": "same resident tables and kernels, one request per launch sequence on one stream, "
+                         "waiting for each before the next"}
+    res = {}
+    for mode, plain in (("served", False), ("plain", True)):
+        with S.Server(plain=plain) as srv:
+            r, st = srv.replay(reqs, arr)
+            _, sat = srv.replay(reqs, np.zeros(n))
+        res[mode] = r
+        out[mode] = {"qps": st["qps"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+                     "max_ms": st["max_ms"], "ticks": st["ticks"], "max_tick": st["max_tick"],
+                     "saturated_qps": sat["qps"], "warmup_s": st["warmup_s"],
+                     "errors": int((r["status"] != 0).sum())}
+    a, b = res["served"], res["plain"]
+    out["identical"] = bool(np.array_equal(a["status"], b["status"]) and np.array_equal(a["value"], b["value"])
+                            and np.array_equal(a["first"], b["first"]) and np.array_equal(a["last"], b["last"])
+                            and np.array_equal(a["hist"], b["hist"]))
+    out["p99_ratio_plain_over_served"] = out["plain"]["p99_ms"] / out["served"]["p99_ms"]
+    out["saturated_speedup"] = out["served"]["saturated_qps"] / out["plain"]["saturated_qps"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +154,9 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=1e11, help="N of the CPU baseline sample")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=8000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the sieve shards by "
@@ -150,6 +234,10 @@ def main():
                      "algorithmic_ops_per_launch": algo_ops, "crossings": C,
                      "note": "LDS-resident sieve: HBM carries only tables and per-segment summaries"},
     }
+    if not args.no_batch:
+        line["batch"] = bench_batch(st, reps=5)
+    if not args.no_serve:
+        line["serve"] = bench_serve(args.serve_requests)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(int(args.cpu_sample), args.cpu_reps)
     print(json.dumps(line))

@@ -10,10 +10,11 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-# N values that exercise segment edges (30 * 128 KiB = 3,932,160 integers per
-# default segment), the p*p rule of low segments and partial last bytes.
+# N values that exercise segment edges (30 * 144 KiB = 4,423,680 integers per
+# default segment; 30 * 128 KiB = 3,932,160 for the smaller tuned shapes), the
+# p*p rule of low segments and partial last bytes.
 EDGE_N = [7, 8, 10, 29, 30, 31, 100, 101, 1000, 30 * 4096 - 1, 30 * 4096, 3932159, 3932160, 3932161,
-          3932160 * 2 + 17, 10**6, 10**7, 12345677, 10**8]
+          3932160 * 2 + 17, 4423679, 4423680, 4423681, 4423680 * 2 + 17, 10**6, 10**7, 12345677, 10**8]
 
 
 @pytest.mark.parametrize("N", EDGE_N + [2, 3, 4, 5, 6, 10**9])

@@ -1,0 +1,68 @@
+"""The large sweeps: full statistics over [2, 1e11] against the oracle run on
+the same box, [2, 1e12] against the committed oracle fixture (bins 2..1500,
+mod-4/mod-3 tallies every 1e10, records, gaps >= 500), published known
+answers, and the Goldbach check to 1e9 with partition counts at multiples
+of 1e6."""
+
+import json
+from pathlib import Path
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIX = Path(__file__).parent / "golden" / "sweep_1e12.json"
+
+
+def test_full_stats_1e11_vs_oracle(voss, oracle):
+    N = 10**11
+    g = voss.stats.full_stats(N, cp_interval=10**9)
+    o = oracle.full_stats(N, cp_interval=10**9)
+    for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"):
+        assert g[k] == o[k], k
+    assert g["hist"] == o["hist"]
+    assert g["records"] == o["records"]
+    assert g["large_gaps"] == o["large_gaps"]
+    assert g["checkpoints"] == o["checkpoints"]
+    # published values: pi(1e11) (A006880), twin count (A007508), the last
+    # maximal gap below 1e11 (464 after 42,652,618,343; A002386/A005250)
+    assert g["prime_count"] == 4_118_054_813
+    assert g["hist"][2] == 224_376_048
+    assert g["records"][-1] == (42_652_618_343 + 464, 464)
+
+
+def _fixture():
+    if not FIX.exists():
+        pytest.skip("tests/golden/sweep_1e12.json not written")
+    return json.loads(FIX.read_text())
+
+
+def test_sweep_1e12_vs_fixture(voss):
+    f = _fixture()
+    N = f["N"]
+    g = voss.stats.full_stats(N, cp_interval=f["checkpoint_interval"])
+    for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"):
+        assert g[k] == f[k], k
+    hist = {str(i): c for i, c in enumerate(g["hist"][:1501]) if c}
+    assert hist == f["hist_1_1500"]
+    assert [list(r) for r in g["records"]] == f["records"]
+    assert [list(r) for r in g["large_gaps"]] == f["large_gaps_ge_500"]
+    assert [list(c) for c in g["checkpoints"]] == f["checkpoints"]
+    # published: pi(1e12) = 37,607,912,018; 11 gaps >= 500, the largest 540
+    # ending at 738,832,928,467 (library/tests/test_primes.py:447-458)
+    assert g["prime_count"] == 37_607_912_018
+    assert len(g["large_gaps"]) == 11 and g["large_gaps"][0] == (738_832_928_467, 540)
+
+
+def test_goldbach_1e9(voss):
+    f = _fixture()
+    r = voss.conjectures.test_goldbach_upto(10**9)
+    assert r["status"] == "no_counterexample_found_upto_N"
+    assert f["goldbach_counterexamples_upto_1e9"] == 0
+    ns = [j * 10**6 for j in range(1, 1001)]
+    assert voss.stats.goldbach_counts(ns) == f["goldbach_counts"]
+
+
+def test_prime_count_1e13(voss):
+    # library/STATE.md:102-103: pi(1e13) = 346,065,536,839
+    assert voss.primes.prime_count(10**13) == 346_065_536_839
