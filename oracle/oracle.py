@@ -28,11 +28,16 @@ _lib = None
 _ref = None
 
 
+def _stale(so: Path) -> bool:
+    srcs = [_HERE / "voss_oracle.c", _HERE / "voss_oracle.h"]
+    return not so.exists() or any(s.stat().st_mtime > so.stat().st_mtime for s in srcs)
+
+
 def build(force: bool = False) -> None:
     """Compile liboracle.so (and _ref/ when the reference tree exists)."""
     import subprocess
     so = _HERE / "liboracle.so"
-    if force or not so.exists():
+    if force or _stale(so):
         subprocess.run(["make", "-C", str(_HERE), "-B", "liboracle.so"], check=True,
                        stdout=subprocess.DEVNULL)
     if Path("/root/reference/library").exists():
@@ -43,7 +48,7 @@ def lib():
     global _lib
     if _lib is None:
         so = _HERE / "liboracle.so"
-        if not so.exists():
+        if _stale(so):
             build()
         L = ctypes.CDLL(str(so))
         u64, u32, i32 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
@@ -70,6 +75,8 @@ def lib():
         L.orc_goldbach_counts.restype = i32
         L.orc_goldbach_test_upto.argtypes = [u64, P64, u64, i32]
         L.orc_goldbach_test_upto.restype = u64
+        L.orc_goldbach_test_upto_pmax.argtypes = [u64, u64, P64, u64, i32]
+        L.orc_goldbach_test_upto_pmax.restype = u64
         L.orc_sophie_germain.argtypes = [u64]
         L.orc_sophie_germain.restype = u64
         _lib = L
@@ -178,9 +185,14 @@ def goldbach_counts(ns, threads=0):
     return list(out[:k])
 
 
-def goldbach_test_upto(N, max_ce=10, threads=0):
+def goldbach_test_upto(N, max_ce=10, threads=0, pmax=None):
+    """(number of even n in [4, N] with no partition, the first max_ce of them).
+    pmax restricts partners to p <= pmax (a planted defect for pinning)."""
     buf = (ctypes.c_uint64 * max(1, max_ce))()
-    n = lib().orc_goldbach_test_upto(N, buf, max_ce, threads)
+    if pmax is None:
+        n = lib().orc_goldbach_test_upto(N, buf, max_ce, threads)
+    else:
+        n = lib().orc_goldbach_test_upto_pmax(N, pmax, buf, max_ce, threads)
     return n, list(buf[:min(n, max_ce)])
 
 

@@ -601,45 +601,66 @@ int orc_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out, int threa
 /* goldbach_upto.cu:25-46: for each even n in [4, N], does some prime
  * p <= n/2 have n - p prime?  Counterexamples ascending. */
 #define GBT_BLOCK (1ull << 20)
-#define GBT_KEEP 16
-typedef struct { const oddbm* B; uint64_t N; uint64_t* found; uint32_t* nfound; atomic_ulong next; uint64_t blocks; } gbt_job;
+typedef struct {
+    const oddbm* B; uint64_t N, pmax, keep;
+    uint64_t** found; uint64_t* nfound; atomic_ulong next; uint64_t blocks;
+} gbt_job;
 static void* gbt_worker(void* a) {
     gbt_job* J = (gbt_job*)a;
     for (;;) {
         uint64_t blk = atomic_fetch_add(&J->next, 1);
         if (blk >= J->blocks) break;
         uint64_t n0 = 6 + blk * GBT_BLOCK * 2;
-        uint32_t nf = 0;
+        uint64_t nf = 0, cap = 0;
+        uint64_t* list = NULL;
         for (uint64_t i = 0; i < GBT_BLOCK; ++i) {
             uint64_t n = n0 + 2 * i;
             if (n > J->N) break;
             int ok = 0;
-            for (uint64_t p = 3; p <= n / 2; p += 2)
+            for (uint64_t p = 3; p <= n / 2 && p <= J->pmax; p += 2)
                 if (oddbm_get(J->B, p) && oddbm_get(J->B, n - p)) { ok = 1; break; }
-            if (!ok) { if (nf < GBT_KEEP) J->found[blk * GBT_KEEP + nf] = n; nf++; }
+            if (ok) continue;
+            if (nf < J->keep) {  /* the block's first `keep` counterexamples, ascending */
+                if (nf == cap) {
+                    cap = cap ? 2 * cap : 16;
+                    list = (uint64_t*)realloc(list, cap * sizeof(uint64_t));
+                }
+                list[nf] = n;
+            }
+            nf++;
         }
+        J->found[blk] = list;
         J->nfound[blk] = nf;
     }
     return NULL;
 }
 uint64_t orc_goldbach_test_upto(uint64_t N, uint64_t* ce, uint64_t max_ce, int threads) {
+    return orc_goldbach_test_upto_pmax(N, UINT64_MAX, ce, max_ce, threads);
+}
+
+/* Same scan with partners p restricted to p <= pmax: a planted defect whose
+ * counterexamples a brute force can list, so the scan and its reporting are
+ * pinned even though the true answer below 1e9 is "none". */
+uint64_t orc_goldbach_test_upto_pmax(uint64_t N, uint64_t pmax, uint64_t* ce, uint64_t max_ce, int threads) {
     if (N < 6) return 0; /* 4 = 2 + 2 */
     oddbm B = oddbm_build(N);
-    gbt_job J; J.B = &B; J.N = N;
+    gbt_job J; J.B = &B; J.N = N; J.pmax = pmax; J.keep = max_ce;
     J.blocks = ((N - 6) / 2) / GBT_BLOCK + 1;
-    J.found = (uint64_t*)calloc(J.blocks * GBT_KEEP, sizeof(uint64_t));
-    J.nfound = (uint32_t*)calloc(J.blocks, sizeof(uint32_t));
+    J.found = (uint64_t**)calloc(J.blocks, sizeof(uint64_t*));
+    J.nfound = (uint64_t*)calloc(J.blocks, sizeof(uint64_t));
     atomic_init(&J.next, 0);
     int T = resolve_threads(threads);
     pthread_t* th = (pthread_t*)malloc(T * sizeof(pthread_t));
     for (int t = 0; t < T; ++t) pthread_create(&th[t], NULL, gbt_worker, &J);
     for (int t = 0; t < T; ++t) pthread_join(th[t], NULL);
     uint64_t n = 0;
-    for (uint64_t b = 0; b < J.blocks; ++b)
-        for (uint32_t i = 0; i < J.nfound[b]; ++i) {
-            if (i < GBT_KEEP && n < max_ce && ce) ce[n] = J.found[b * GBT_KEEP + i];
+    for (uint64_t b = 0; b < J.blocks; ++b) {
+        for (uint64_t i = 0; i < J.nfound[b]; ++i) {
+            if (i < J.keep && n < max_ce && ce) ce[n] = J.found[b][i];
             n++;
         }
+        free(J.found[b]);
+    }
     free(th); free(J.found); free(J.nfound); free(B.w);
     return n;
 }

@@ -90,6 +90,7 @@ uint64_t orc_goldbach_count_sieve(uint64_t n);         /* same count from a siev
 int orc_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out, int threads);
 /* Even n in [4, N] with no partition, ascending, at most max_ce (goldbach_upto.cu:25-46). */
 uint64_t orc_goldbach_test_upto(uint64_t N, uint64_t* ce, uint64_t max_ce, int threads);
+uint64_t orc_goldbach_test_upto_pmax(uint64_t N, uint64_t pmax, uint64_t* ce, uint64_t max_ce, int threads);
 /* Sophie Germain primes p <= N (p and 2p+1 prime) (sophie_germain_upto.cu:22-33). */
 uint64_t orc_sophie_germain(uint64_t N);
 

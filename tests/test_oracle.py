@@ -214,6 +214,14 @@ def test_goldbach_vs_reference_cpu(oracle):
 
 def test_goldbach_upto_and_sophie(oracle):
     assert oracle.goldbach_test_upto(10**6) == (0, [])
+    # planted defect: partners restricted to p <= pmax give counterexamples a
+    # brute force lists exactly (count, order, the first max_ce, block edges)
+    ps = set(simple_primes(3 * 10**6))
+    for N, pmax, cap in [(2000, 13, 10), (3 * 10**6, 31, 40), (3 * 10**6, 7, 10**6)]:
+        want = [n for n in range(6, N + 1, 2)
+                if not any(p in ps and (n - p) in ps for p in range(3, min(n // 2, pmax) + 1, 2))]
+        got_n, got = oracle.goldbach_test_upto(N, max_ce=cap, pmax=pmax)
+        assert got_n == len(want) and got == want[:cap], (N, pmax)
     assert [oracle.sophie_germain(n) for n in (10, 30, 1000, 100, 10**6)] == [3, 6, 37, 10, 7746]
 
 
