@@ -85,7 +85,8 @@ struct SieveArgs {
   uint32_t nseg;                // segments in this launch
   uint32_t seg_base;            // global index of this launch's segment 0
   uint32_t tier1_end;           // primes [0, tier1_end): one prime per wave, lanes split the segment
-  uint32_t tier2_end;           // primes [tier1_end, tier2_end): one per lane, unrolled walk; rest: one per lane, <= 4 hits/class
+  uint32_t tier2_end;           // primes [tier1_end, tier2_end): one per lane, unrolled walk
+  uint32_t tier3_end;           // primes [tier2_end, tier3_end): one per lane, <= 4 hits/class; rest (p >= seg_bytes): <= 1
   unsigned int* work;           // batch mode: work counter (zeroed before launch)
   const uint32_t* nseg_dev;     // batch mode: if set, the segment count is read here
   // outputs (mode dependent; unused ones may be null)

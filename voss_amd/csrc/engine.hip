@@ -261,6 +261,7 @@ SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   A.nseg = nseg;
   A.tier1_end = tier_index(c.tier1_p);
   A.tier2_end = tier_index(c.seg_bytes / 4);
+  A.tier3_end = tier_index(c.seg_bytes);
   A.large_threshold = 0xFFFFFFFFu;
   return A;
 }

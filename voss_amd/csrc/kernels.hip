@@ -260,14 +260,49 @@ __device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint3
   return offl;
 }
 
+// Clear bit `bit` of segment byte off.
+__device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t bit) {
+  const uint32_t a = base + off;
+  lds_and_at(a & ~3u, ~(1u << (((a & 3) << 3) | bit)));
+}
+
+// Every crossing of p (rk = k0 mod p) in the segment, class by class, from
+// two packed 64-bit LDS rows: off = min(cc + p - rk, cc - rk), the p*p rule
+// applied where it can bite.  Suits primes with a few crossings per class.
+__device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
+                                                    uint32_t nbytes, const uint8_t* tab) {
+  const uint32_t q30 = p / 30;
+  const uint32_t prk = p - rk;
+  const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
+  const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+  const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;  // the p*p rule can bite
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
+    uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
+    if (low) {
+      const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
+      const uint64_t fb = (uint64_t)p * m / 30;
+      if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
+    }
+    const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
+    while (off < nbytes) {
+      cross_one(base, off, bit);
+      off += p;
+    }
+  }
+}
+
 // Cross off every sieving prime >= 89 with p*p <= the segment's top value.
 //   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
 //     lanes are 8 classes x 8 sub-ranges of the segment;
 //   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
 //     unrolled loop;
-//   tier 3 (p >= seg_bytes/4): one prime per lane, at most four crossings per
-//     class, so the per-class setup is kept to a few instructions: class
-//     offsets from two packed 64-bit LDS rows and off = min(cc + p - rk, cc - rk).
+//   tier 3 (seg_bytes/4 <= p < seg_bytes): one prime per lane, at most four
+//     crossings per class, so the per-class setup is kept to a few
+//     instructions (cross_prime_classes);
+//   tier 4 (p >= seg_bytes): one prime per lane, only the classes whose single
+//     crossing falls inside the segment are visited.
 template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
@@ -285,6 +320,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   const uint32_t np = A.pi_tab[ti];
   const uint32_t n1 = min(np, A.tier1_end);
   const uint32_t n2 = max(n1, min(np, A.tier2_end));
+  const uint32_t n3 = max(n2, min(np, A.tier3_end));
 
   {
     const uint32_t c = lane & 7, sr = lane >> 3;
@@ -316,31 +352,51 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (offl < nbytes) cross_class(base, (uint32_t)offl, p, nbytes, tab[64 + r * 8 + c]);
     }
   }
-  for (uint32_t b = n2 + 64 * wave; b < np; b += 64 * NW) {
+  for (uint32_t b = n2 + 64 * wave; b < n3; b += 64 * NW) {
+    const uint32_t i = b + lane;
+    if (i >= n3) continue;
+    const uint32_t e = A.primes[i];
+    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+    cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab);
+  }
+  // tier 4 (p >= seg_bytes): at most one crossing per class.  The classes'
+  // first multiples cc_c (byte offsets within one period p) increase with c,
+  // so the classes that hit the window [rk, rk + nbytes) mod p form one run
+  // (two when the window wraps past p): a three-step search finds its start
+  // and only the run is visited.
+  for (uint32_t b = n3 + 64 * wave; b < np; b += 64 * NW) {
     const uint32_t i = b + lane;
     if (i >= np) continue;
     const uint32_t e = A.primes[i];
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     const uint32_t rk = mod_u64(k0, p);
+    if (30 * k0 < (uint64_t)p * p + 30ull * p) {  // the p*p rule can bite: general path
+      cross_prime_classes(base, p, r, rk, k0, nbytes, tab);
+      continue;
+    }
     const uint32_t q30 = p / 30;
-    const uint32_t prk = p - rk;
-    const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
-    const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
-    const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;  // the p*p rule can bite
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
-      uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
-      if (low) {
-        const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
-        const uint64_t fb = (uint64_t)p * m / 30;
-        if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
-      }
-      const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
-      while (off < nbytes) {
-        const uint32_t a = base + off;
-        lds_and_at(a & ~3u, ~(1u << (((a & 3) << 3) | bit)));
-        off += p;
+    const uint32_t* frow = reinterpret_cast<const uint32_t*>(tab + r * 8);
+    const uint32_t* brow = reinterpret_cast<const uint32_t*>(tab + 64 + r * 8);
+    const uint32_t flo = frow[0], fhi = frow[1], blo = brow[0], bhi = brow[1];
+    auto cc = [&](uint32_t c) { return q30 * w30(c) + __builtin_amdgcn_perm(fhi, flo, c | 0x0C0C0C00u); };
+    auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(bhi, blo, c | 0x0C0C0C00u) & 7u; };
+    uint32_t c = 0;  // number of classes with cc_c < rk
+    if (cc(3) < rk) c = 4;
+    if (cc(c + 1) < rk) c += 2;
+    if (cc(c) < rk) c += 1;
+    if (c == 7 && cc(7) < rk) c = 8;
+    const uint32_t end = rk + nbytes;
+    for (; c < 8; ++c) {
+      const uint32_t x = cc(c);
+      if (x >= end) break;
+      cross_one(base, x - rk, bit_of(c));
+    }
+    if (end > p) {
+      const uint32_t end2 = end - p;
+      for (uint32_t c2 = 0; c2 < 8; ++c2) {
+        const uint32_t x = cc(c2);
+        if (x >= end2) break;
+        cross_one(base, x + p - rk, bit_of(c2));
       }
     }
   }
@@ -567,19 +623,12 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   }
 }
 
-// Flush the LDS histograms: gap bins directly, byte values through their
-// internal gaps.  sink(bin, count) receives gap/2 bins.
+// Flush the LDS histograms: byte values are first folded into the gap bins
+// through their internal gaps (LDS atomics), then each non-zero gap bin goes
+// to sink(bin, count) once, so a flush costs at most kHistBins global
+// atomics.  Called by every thread of the workgroup (it has barriers).
 template <int BS, typename Sink>
 __device__ void flush_histograms(uint32_t* lds, Sink sink) {
-  for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
-    uint32_t s = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < kHistCopies; ++c) {
-      s += lds[kLdsHist + h * kHistCopies + c];
-      lds[kLdsHist + h * kHistCopies + c] = 0;
-    }
-    if (s) sink(h, s);
-  }
   for (uint32_t x = threadIdx.x; x < 256; x += BS) {
     uint32_t s = 0;
 #pragma unroll
@@ -588,7 +637,18 @@ __device__ void flush_histograms(uint32_t* lds, Sink sink) {
       lds[kLdsByteHist + x * kHistCopies + c] = 0;
     }
     if (!s) continue;
-    for (uint32_t nib = internal_gaps(x); nib; nib >>= 4) sink(nib & 15, s);
+    for (uint32_t nib = internal_gaps(x); nib; nib >>= 4)
+      lds_add(lds + kLdsHist + (nib & 15) * kHistCopies + (x & (kHistCopies - 1)), s);
+  }
+  __syncthreads();
+  for (uint32_t h = threadIdx.x; h < kHistBins; h += BS) {
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kHistCopies; ++c) {
+      s += lds[kLdsHist + h * kHistCopies + c];
+      lds[kLdsHist + h * kHistCopies + c] = 0;
+    }
+    if (s) sink(h, s);
   }
 }
 
