@@ -7,14 +7,14 @@ start-up:
 
   sweep  full statistics over [2, N] (default N = 1e11, the Context workload)
   batch  a sample of the batched-range workload (log-uniform a in
-         [1e6, 1e13], width in [1e3, 1e9]; default 1024 queries, seed 0)
+         [1e6, 1e13], width in [1e3, 1e9]; default 4096 queries, seed 0)
 
 The search is coordinate-wise: the segment/block/occupancy shape first (LDS
 permitting: segment LDS x workgroups per CU <= 160 KiB), then the tier-1
 bound for the best shape.  Every candidate's result is checked against the
 first one bit for bit, so a configuration can only win by being faster.
 
-    python -m voss_amd.autotune [--n 1e11] [--queries 1024] [--out PATH]
+    python -m voss_amd.autotune [--n 1e11] [--queries 4096] [--out PATH]
 """
 
 import argparse
@@ -63,8 +63,8 @@ def tune(kind, N=10**11, nq=1024, reps=2, log=print):
         qa, qb = _queries(nq)
 
         def run():
-            res = stats.batch_ranges(qa, qb)
-            return tuple((r["count"], r["first"], r["last"], sum(r["hist"])) for r in res)
+            res = stats.batch_ranges(qa, qb, as_numpy=True)
+            return (res["count"].tobytes(), res["first"].tobytes(), res["last"].tobytes(), res["hist"].tobytes())
     ref = None
     results = []
 
@@ -107,7 +107,7 @@ def tune(kind, N=10**11, nq=1024, reps=2, log=print):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=float, default=1e11)
-    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--out", default=None, help="tuned.cfg path (default: next to libvoss.so)")
     ap.add_argument("--log", default=None, help="also write every trial as JSON here")
