@@ -181,6 +181,30 @@ def test_mr_batch(voss, oracle):
     assert got == [oracle.is_prime(x) for x in xs]
 
 
+def test_mr_batch_pseudoprimes_and_dense_primes(voss, oracle):
+    # strong pseudoprimes to base 2, Carmichael numbers, spsp to bases 2..17
+    # and 2..23, the 2^32 edge, and runs made only of primes (every lane of a
+    # wave survives base 2, so the shared second stage takes several rounds)
+    spsp2 = [2047, 3277, 4033, 4681, 8321, 15841, 29341, 42799, 49141, 52633, 65281, 74665, 80581, 85489,
+             88357, 90751, 1194649, 12327121]
+    carmichael = [561, 1105, 1729, 2465, 2821, 6601, 8911, 41041, 62745, 63973, 75361, 101101, 126217,
+                  172081, 188461, 252601, 278545, 294409, 314821, 334153, 340561, 399001, 410041, 449065]
+    big = [341550071728321, 3825123056546413051, 4294967291, 4294967293, 4294967295, 4294967296,
+           4294967297, 4294967311, 2**63 - 25, 2**63 + 29, 2**64 - 59, 2**64 - 95]
+    rng = random.Random(11)
+    primes = []
+    for _ in range(200):
+        rc, q = oracle.next_prime(rng.getrandbits(63) + 2**63 - 2**62)
+        if rc == 0:
+            primes.append(q)
+    small_primes = [oracle.next_prime(rng.getrandbits(31))[1] for _ in range(100)]
+    semis = [a * b for a, b in zip(small_primes[:50], small_primes[50:])]
+    xs = spsp2 + carmichael + big + primes + small_primes + semis
+    got = voss.stats.is_prime_batch(xs)
+    assert got == [oracle.is_prime(x) for x in xs]
+    assert all(got[len(spsp2) + len(carmichael) + len(big):][:len(primes) + len(small_primes)])
+
+
 def test_next_prev_batch(voss, oracle):
     rng = random.Random(5)
     xs = [0, 1, 2, 3, 4, 10, 100, 10**9, 10**12] + [rng.getrandbits(rng.randint(2, 62)) for _ in range(300)]

@@ -79,48 +79,102 @@ __device__ Mont mont_setup(uint64_t n) {
   return M;
 }
 
-__device__ bool mr_is_prime(uint64_t n) {
-  if (n < 2) return false;
-  if (n == 2 || n == 3) return true;
-  if ((n & 1) == 0) return false;
+// Deterministic strong-probable-prime bases: {2, 7, 61} decide every n < 2^32
+// (Jaeschke 1993); the seven bases below decide every n < 2^64 (Sinclair 2011).
+// Primality is a fact of n, so the answers equal those of the reference's
+// twelve bases 2..37 (core/miller_rabin.cpp:26-51); fewer bases only make
+// the primes, which must pass every base, cheaper.
+__constant__ uint32_t c_bases32[3] = {2, 7, 61};
+__constant__ uint32_t c_bases64[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+
+// 0: composite, 1: prime, 2: odd and free of prime factors <= 37 (needs MR).
+__device__ __forceinline__ int mr_prescreen(uint64_t n) {
+  if (n < 2) return 0;
+  if (n == 2 || n == 3) return 1;
+  if ((n & 1) == 0) return 0;
   const uint32_t small[11] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
   for (int i = 0; i < 11; ++i) {
-    if (n == small[i]) return true;
-    if (n % small[i] == 0) return false;
+    if (n == small[i]) return 1;
+    if (n % small[i] == 0) return 0;
   }
-  uint64_t d = n - 1;
-  int r = 0;
-  while ((d & 1) == 0) { d >>= 1; ++r; }
-  const Mont M = mont_setup(n);
+  return n < 41 * 41 ? 1 : 2;
+}
+
+// Strong probable-prime test of odd n = d * 2^r + 1 to base a.
+__device__ bool mr_strong(const Mont& M, uint64_t d, int r, uint64_t a) {
+  if (a >= M.n) a %= M.n;
+  if (a == 0) return true;
   const uint64_t minus1 = M.n - M.one;  // -1 in Montgomery form
-  const uint32_t wit[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-  for (int i = 0; i < 12; ++i) {
-    const uint64_t a = wit[i];
-    if (a >= n) continue;
-    uint64_t x = M.one, base = mont_mul(a, M.r2, M);
-    uint64_t e = d;
-    while (e) {
-      if (e & 1) x = mont_mul(x, base, M);
-      base = mont_mul(base, base, M);
-      e >>= 1;
-    }
-    if (x == M.one || x == minus1) continue;
-    bool comp = true;
-    for (int k = 0; k < r - 1; ++k) {
-      x = mont_mul(x, x, M);
-      if (x == minus1) { comp = false; break; }
-    }
-    if (comp) return false;
+  uint64_t x = M.one, base = mont_mul(a, M.r2, M);
+  for (uint64_t e = d; e; e >>= 1) {
+    if (e & 1) x = mont_mul(x, base, M);
+    base = mont_mul(base, base, M);
   }
+  if (x == M.one || x == minus1) return true;
+  for (int k = 0; k < r - 1; ++k) {
+    x = mont_mul(x, x, M);
+    if (x == minus1) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ uint32_t mr_base(uint64_t n, uint32_t i) {
+  return n < (1ull << 32) ? (i < 3 ? c_bases32[i] : 0u) : c_bases64[i];
+}
+
+// One lane, every base in turn (composites usually stop at base 2).
+__device__ bool mr_is_prime(uint64_t n) {
+  const int pre = mr_prescreen(n);
+  if (pre != 2) return pre == 1;
+  uint64_t d = n - 1;
+  const int r = __builtin_ctzll(d);
+  d >>= r;
+  const Mont M = mont_setup(n);
+  const uint32_t nb = n < (1ull << 32) ? 3 : 7;
+  for (uint32_t i = 0; i < nb; ++i)
+    if (!mr_strong(M, d, r, mr_base(n, i))) return false;
   return true;
 }
 
 }  // namespace
 
 // ---------------------------------------------------------------- kernels
-__global__ void is_prime_kernel(const uint64_t* x, uint64_t n, uint8_t* out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = mr_is_prime(x[i]) ? 1 : 0;
+// One wave per 64 values.  Stage 1: every undecided value takes base 2 on its
+// own lane.  Stage 2: the survivors' remaining bases (six each) are spread
+// over the wave's lanes, so a wave holding a prime spends about two
+// exponentiations instead of seven in sequence.
+__global__ void __launch_bounds__(64) is_prime_kernel(const uint64_t* x, uint64_t n, uint8_t* out) {
+  __shared__ uint64_t s_val[64];
+  __shared__ uint32_t s_comp[64];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t i = (uint64_t)blockIdx.x * 64 + lane;
+  const uint64_t v = i < n ? x[i] : 0;
+  int st = mr_prescreen(v);
+  if (st == 2) {
+    uint64_t d = v - 1;
+    const int r = __builtin_ctzll(d);
+    d >>= r;
+    if (!mr_strong(mont_setup(v), d, r, 2)) st = 0;
+  }
+  const uint64_t surv = __ballot(st == 2);
+  const uint32_t ord = __builtin_amdgcn_mbcnt_hi((uint32_t)(surv >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)surv, 0u));
+  const uint32_t m = (uint32_t)__builtin_popcountll(surv);  // wave-uniform
+  if (st == 2) s_val[ord] = v;
+  s_comp[lane] = 0;
+  __syncthreads();
+  for (uint32_t t = lane; t < 6 * m; t += 64) {
+    const uint32_t k = t / 6, b = t % 6 + 1;
+    const uint64_t nv = s_val[k];
+    const uint32_t a = mr_base(nv, b);
+    if (!a) continue;
+    uint64_t d = nv - 1;
+    const int r = __builtin_ctzll(d);
+    d >>= r;
+    if (!mr_strong(mont_setup(nv), d, r, a)) s_comp[k] = 1;
+  }
+  __syncthreads();
+  if (st == 2) st = s_comp[ord] ? 0 : 1;
+  if (i < n) out[i] = (uint8_t)st;
 }
 
 // next: smallest prime > x (x < 2 -> 2); prev: largest prime < x (x <= 2 -> 0).
@@ -286,7 +340,7 @@ __global__ void sophie_kernel(const uint8_t* bm, uint64_t N, unsigned long long*
 // ---------------------------------------------------------------- launchers
 hipError_t launch_is_prime(const uint64_t* x, uint64_t n, uint8_t* out, hipStream_t st) {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(is_prime_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, x, n, out);
+  hipLaunchKernelGGL(is_prime_kernel, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, st, x, n, out);
   return hipGetLastError();
 }
 hipError_t launch_next_prev(const uint64_t* x, uint64_t n, uint64_t* out, int dir, hipStream_t st) {
