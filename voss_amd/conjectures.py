@@ -2,15 +2,20 @@
 dictionaries as bindings/python/voss/conjectures.py).  They never prove
 anything: "no counterexample found" is evidence up to N only.
 
-test_goldbach_upto / iter_goldbach_upto run on the GPU
-(voss_primes_goldbach_test_upto); the others are loops over next_prime and
-in_range, as in the reference.
+Where the reference loops over next_prime / in_range one value at a time
+(library/bindings/python/voss/conjectures.py), these run on the GPU in bulk:
+Bertrand and the Chebyshev bias from one prime list, Legendre from one
+batched next-prime launch over n^2, Goldbach through
+voss_primes_goldbach_test_upto.  The answers (and the first ten
+counterexamples, ascending) are the ones the per-value loops give.
 """
 
 import ctypes
 import math
 
-from . import _capi, primes
+import numpy as np
+
+from . import _capi, primes, stats
 
 _NOTE = "Empirical test up to N. Does NOT prove the conjecture."
 
@@ -43,31 +48,45 @@ def _goldbach_counterexamples(N, limit=10):
     return _capi.take_u64(arr, cnt.value)
 
 
+def _prime_array(hi):
+    """All primes <= hi as a uint64 array (the GPU list; 2, 3, 5 prepended)."""
+    head = np.array([q for q in (2, 3, 5) if q <= hi], dtype=np.uint64)
+    if hi < 7:
+        return head
+    return np.concatenate([head, primes._in_range_np(7, hi)])
+
+
+def _next_primes(lo, hi, P):
+    """next_prime(n) for n in [lo, hi] from an ascending prime array P reaching past hi."""
+    n = np.arange(lo, hi + 1, dtype=np.uint64)
+    return n, P[np.searchsorted(P, n, side="right")]
+
+
 def test_bertrand_upto(N):
     """For every n in [2, N] (N <= 1e7): next_prime(n) < 2n."""
     _validate_N(N, 2, 10**7)
-    bad = []
-    for n in range(2, N + 1):
-        if primes.next_prime(n) >= 2 * n:
-            bad.append(n)
-            if len(bad) >= 10:
-                break
-    return _result("bertrand", N, bad)
+    n, nxt = _next_primes(2, N, _prime_array(2 * N + 2))
+    bad = n[nxt >= 2 * n][:10]
+    return _result("bertrand", N, [int(v) for v in bad])
+
+
+def _legendre_next(N):
+    """(n, next_prime(n^2)) for n = 1 .. isqrt(N) - 1 (batched GPU next-prime)."""
+    n = np.arange(1, math.isqrt(N), dtype=np.uint64)
+    nxt = np.empty_like(n)
+    step = 1 << 20
+    for i in range(0, len(n), step):
+        sq = n[i:i + step] * n[i:i + step]
+        nxt[i:i + step] = stats.next_prime_batch(sq, as_numpy=True)
+    return n, nxt
 
 
 def test_legendre_upto(N):
     """For every n with (n+1)^2 <= N (N <= 1e14): a prime lies in (n^2, (n+1)^2)."""
     _validate_N(N, 2, 10**14)
-    bad = []
-    for n in range(1, math.isqrt(N)):
-        upper = (n + 1) * (n + 1)
-        if upper > N:
-            break
-        if primes.next_prime(n * n) >= upper:
-            bad.append(n)
-            if len(bad) >= 10:
-                break
-    return _result("legendre", N, bad)
+    n, nxt = _legendre_next(N)
+    bad = n[nxt >= (n + 1) * (n + 1)][:10]
+    return _result("legendre", N, [int(v) for v in bad])
 
 
 def test_goldbach_upto(N):
@@ -76,53 +95,46 @@ def test_goldbach_upto(N):
     return _result("goldbach", N, _goldbach_counterexamples(N))
 
 
+def _chebyshev_counts(N):
+    # the reference walks primes.in_range(2, N), which for N > 100 omits 2, 3
+    # and 5 (library/src/primes/range.cu:130-134); the same list is used here
+    P = primes._in_range_np(2, N)
+    P = P[P != 2]
+    c1 = np.cumsum(P % 4 == 1, dtype=np.int64)
+    c3 = np.arange(1, len(P) + 1, dtype=np.int64) - c1
+    return P, c1, c3
+
+
 def test_chebyshev_bias_upto(N):
     """Strict Chebyshev bias pi(x;4,3) > pi(x;4,1) for every prime x <= N (N <= 1e7)."""
     _validate_N(N, 3, 10**7)
-    bad = []
-    c1 = c3 = 0
-    for p in primes.in_range(2, N):
-        if p == 2:
-            continue
-        if p % 4 == 1:
-            c1 += 1
-        else:
-            c3 += 1
-        if c3 <= c1:
-            bad.append(p)
-            if len(bad) >= 10:
-                break
-    return _result("chebyshev_bias", N, bad)
+    P, c1, c3 = _chebyshev_counts(N)
+    bad = P[c3 <= c1][:10]
+    return _result("chebyshev_bias", N, [int(v) for v in bad])
 
 
 def iter_bertrand_upto(N):
     """Yield (n, next_prime(n)) for n in [2, N]."""
     _validate_N(N, 2, 10**7)
-    for n in range(2, N + 1):
-        yield n, primes.next_prime(n)
+    n, nxt = _next_primes(2, N, _prime_array(2 * N + 2))
+    for a, b in zip(n.tolist(), nxt.tolist()):
+        yield a, b
 
 
 def iter_legendre_upto(N):
     """Yield (n, next_prime(n^2)) while (n+1)^2 <= N."""
     _validate_N(N, 2, 10**14)
-    for n in range(1, math.isqrt(N)):
-        if (n + 1) * (n + 1) > N:
-            break
-        yield n, primes.next_prime(n * n)
+    n, nxt = _legendre_next(N)
+    for a, b in zip(n.tolist(), nxt.tolist()):
+        yield a, b
 
 
 def iter_chebyshev_bias_upto(N):
     """Yield (p, c1, c3) for each odd prime p <= N."""
     _validate_N(N, 3, 10**7)
-    c1 = c3 = 0
-    for p in primes.in_range(2, N):
-        if p == 2:
-            continue
-        if p % 4 == 1:
-            c1 += 1
-        else:
-            c3 += 1
-        yield p, c1, c3
+    P, c1, c3 = _chebyshev_counts(N)
+    for a, b, c in zip(P.tolist(), c1.tolist(), c3.tolist()):
+        yield a, b, c
 
 
 def iter_goldbach_upto(N):

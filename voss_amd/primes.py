@@ -277,6 +277,20 @@ def in_range(a: int, b: int) -> list:
     return _capi.take_u64(arr, n)
 
 
+def _in_range_np(a: int, b: int):
+    """in_range as a numpy uint64 array (same primes, no per-element list)."""
+    import numpy as np
+    if b < a:
+        return np.zeros(0, dtype=np.uint64)
+    arr, n = _call_array(_L.voss_primes_in_range, ctypes.c_uint64(a), ctypes.c_uint64(b))
+    if n == 0 or not arr:
+        return np.zeros(0, dtype=np.uint64)
+    try:
+        return np.ctypeslib.as_array(arr, shape=(n,)).copy()
+    finally:
+        _L.voss_free(arr)
+
+
 def in_range_with_limit(a: int, b: int, max_count: int) -> list:
     """in_range with a caller-chosen maximum count."""
     if not isinstance(a, int) or not isinstance(b, int):

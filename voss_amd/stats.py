@@ -121,10 +121,16 @@ def is_prime_batch(xs: list) -> list:
     return [bool(out[i]) for i in range(n)]
 
 
-def next_prime_batch(xs: list) -> list:
+def next_prime_batch(xs, as_numpy: bool = False):
+    """Smallest prime > x per value (GPU; 0 where none fits in uint64).
+    as_numpy returns a numpy uint64 array (xs may be one too)."""
     n = len(xs)
     out = (ctypes.c_uint64 * max(1, n))()
-    _capi._check(_L.voss_amd_next_prime_batch(_capi.u64_array(xs), n, out), _ERRMAP)
+    keep, px = _u64_arg(xs)
+    _capi._check(_L.voss_amd_next_prime_batch(px, n, out), _ERRMAP)
+    if as_numpy:
+        import numpy as np
+        return np.frombuffer(out, dtype=np.uint64, count=n).copy()
     return list(out[:n])
 
 
