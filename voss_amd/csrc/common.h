@@ -10,19 +10,13 @@
 
 namespace voss {
 
-// Primes 7..83 are removed by AND-ing six periodic byte patterns into LDS
-// (pattern g has period = product of its primes, in bytes); sieving proper
-// starts at 89.
-constexpr int kPresieveGroups = 6;
-constexpr uint32_t kPresievePeriods[kPresieveGroups] = {
-    7u * 11u * 13u * 17u * 19u,  // 323,323
-    23u * 29u * 31u,             // 20,677
-    37u * 41u * 43u,             // 65,231
-    47u * 53u * 59u,             // 146,969
-    61u * 67u * 71u,             // 290,177
-    73u * 79u * 83u,             // 478,661
-};
-constexpr uint32_t kFirstSievingPrime = 89;
+// Primes 7..19 are removed by AND-ing a periodic byte pattern (period
+// 7*11*13*17*19 = 323,323 bytes) into LDS; sieving proper starts at 23.
+constexpr int kPresieveGroups = 1;
+constexpr uint32_t kPresievePeriods[kPresieveGroups] = {7u * 11u * 13u * 17u * 19u};
+constexpr uint32_t kFirstSievingPrime = 23;
+// Primes below this are sieved by tier 0 (a half-wave per prime and class).
+constexpr uint32_t kTier0P = 256;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
 constexpr uint32_t kHistBins = 512;   // LDS gap histogram bins (index gap/2)
 constexpr uint32_t kHistCopies = 4;   // copies, one per 16-lane group
@@ -71,7 +65,7 @@ enum KernelMode : int {
 // Arguments of the segment kernel (passed by value).
 struct SieveArgs {
   // resident tables
-  const uint32_t* primes;       // sieving primes >= 89, bits 24..26 = index of p mod 30
+  const uint32_t* primes;       // sieving primes >= 23, bits 24..26 = index of p mod 30
   const uint32_t* pi_tab;       // pi_tab[i] = #sieving primes <= i*kPiTabStep
   uint32_t pi_tab_len;
   const uint32_t* presieve;     // pattern bytes as u32 words, kPresieveGroups patterns
@@ -84,7 +78,8 @@ struct SieveArgs {
   uint32_t seg_bytes;           // S (multiple of 16 * waves * 64... see host)
   uint32_t nseg;                // segments in this launch
   uint32_t seg_base;            // global index of this launch's segment 0
-  uint32_t tier1_end;           // primes [0, tier1_end): one prime per wave, lanes split the segment
+  uint32_t tier0_end;           // primes [0, tier0_end): a half-wave per (prime, class), lanes split the crossings
+  uint32_t tier1_end;           // primes [tier0_end, tier1_end): one prime per wave, lanes split the segment
   uint32_t tier2_end;           // primes [tier1_end, tier2_end): one per lane, unrolled walk
   uint32_t tier3_end;           // primes [tier2_end, tier3_end): one per lane, <= 4 hits/class; rest (p >= seg_bytes): <= 1
   unsigned int* work;           // batch mode: work counter (zeroed before launch)

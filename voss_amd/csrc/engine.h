@@ -181,7 +181,7 @@ class Engine {
   TuneConfig cfgs_[2];
   TuneConfig cfg_;  // the configuration of the call in flight
   // resident tables
-  std::vector<uint32_t> h_primes_;  // sieving primes >= 89 (plain values)
+  std::vector<uint32_t> h_primes_;  // sieving primes >= 23 (plain values)
   uint32_t* d_primes_ = nullptr;
   uint32_t* d_pitab_ = nullptr;
   uint32_t pitab_len_ = 0;
@@ -189,6 +189,7 @@ class Engine {
   uint32_t presieve_off_[kPresieveGroups] = {0};
   uint32_t first_word_ = 0;
   uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
+  uint32_t tier0_p_ = kTier0P;  // primes below: kernel tier 0
   KernelTimes times_;
 };
 

@@ -100,6 +100,7 @@ Engine::Engine() {
   ws_.resize(kWsCount_);
 
   load_tune_file();
+  tier0_p_ = env_u32("VOSS_TIER0_P", kTier0P);
   if (std::getenv("VOSS_SEG_BYTES") || std::getenv("VOSS_BLOCK") || std::getenv("VOSS_WGS_PER_CU") ||
       std::getenv("VOSS_TIER1_P")) {
     TuneConfig c = cfgs_[0];
@@ -114,9 +115,7 @@ Engine::Engine() {
   // presieve patterns: in pattern g, byte j has bit o set iff 30j + W30[o] has
   // no factor among the group's primes; each is stored for one period plus a
   // full segment so a segment never wraps
-  static const uint32_t groups[kPresieveGroups][5] = {
-      {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0}, {37, 41, 43, 0, 0},
-      {47, 53, 59, 0, 0}, {61, 67, 71, 0, 0}, {73, 79, 83, 0, 0}};
+  static const uint32_t groups[kPresieveGroups][5] = {{7, 11, 13, 17, 19}};
   std::vector<uint32_t> words;
   for (int gi = 0; gi < kPresieveGroups; ++gi) {
     presieve_off_[gi] = (uint32_t)words.size();
@@ -259,6 +258,7 @@ SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   A.first_word = first_word_;
   A.seg_bytes = c.seg_bytes;
   A.nseg = nseg;
+  A.tier0_end = tier_index(tier0_p_);
   A.tier1_end = tier_index(c.tier1_p);
   A.tier2_end = tier_index(c.seg_bytes / 4);
   A.tier3_end = tier_index(c.seg_bytes);

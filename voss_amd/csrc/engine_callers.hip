@@ -30,7 +30,7 @@ std::vector<uint64_t> Engine::goldbach_upto(uint64_t N, uint64_t max_ce) {
   // ascending odd primes 3, 5, ..., up to min(N/2, table limit).  An n with no
   // partner among them is only a candidate; candidates are settled below by
   // an exact partition count over every prime <= n/2.
-  std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83};
+  std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19};  // the table starts at 23
   while (!sp.empty() && (uint64_t)sp.back() > N / 2) sp.pop_back();
   for (uint32_t p : h_primes_) {
     if ((uint64_t)p > N / 2) break;

@@ -156,7 +156,7 @@ __device__ void init_tables(uint32_t* lds) {
 }
 
 // Initialise the segment in LDS as the AND of the presieve patterns (primes
-// 7..83 removed), zero-filling up to the next 1 KiB so later passes read whole
+// 7..19 removed), zero-filling up to the next 1 KiB so later passes read whole
 // 16-byte chunks.  Bytes 0..3 of the global bitmap (values below 120) get
 // their true bits: 1 is not prime, and 7..113 must stay set.
 template <int BS>
@@ -244,6 +244,37 @@ __device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t 
   if (jt + 2 * p < end) lds_and_at(a + d2, m2);
 }
 
+// Clear bit `bit` of cnt bytes s, s+p, s+2p, ... (as cross_class, counted).
+__device__ __forceinline__ void cross_run(uint32_t base, uint32_t s, uint32_t p, uint32_t cnt, uint32_t bit) {
+  const uint32_t j0 = s & 3;
+  const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
+  const uint32_t m0 = ~(1u << (j0 * 8 + bit));
+  const uint32_t m1 = ~(1u << (((j0 + p) & 3) * 8 + bit));
+  const uint32_t m2 = ~(1u << (((j0 + 2 * p) & 3) * 8 + bit));
+  const uint32_t m3 = ~(1u << (((j0 + 3 * p) & 3) * 8 + bit));
+  uint32_t a = base + (s & ~3u);
+  const uint32_t step = 4 * p;
+  for (uint32_t g = cnt >> 2; g; --g) {
+    lds_and_at(a, m0);
+    lds_and_at(a + d1, m1);
+    lds_and_at(a + d2, m2);
+    lds_and_at(a + d3, m3);
+    a += step;
+  }
+  const uint32_t t = cnt & 3;
+  if (t > 0) lds_and_at(a, m0);
+  if (t > 1) lds_and_at(a + d1, m1);
+  if (t > 2) lds_and_at(a + d2, m2);
+}
+
+// floor(x / p) for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
+__device__ __forceinline__ uint32_t div_u32(uint32_t x, uint32_t p) {
+  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)p));
+  if (q * p > x) --q;
+  if ((q + 1) * p <= x) ++q;
+  return q;
+}
+
 // First crossing of class c at or after byte k0 of the global bitmap,
 // relative to k0; never below p*p (the reference rule, context.cu:354-388).
 __device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint32_t c, uint32_t rk, uint64_t k0,
@@ -293,7 +324,9 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
   }
 }
 
-// Cross off every sieving prime >= 89 with p*p <= the segment's top value.
+// Cross off every sieving prime >= 23 with p*p <= the segment's top value.
+//   tier 0 (p < 256): a half-wave per (prime, class), its 32 lanes splitting
+//     the class's crossings into runs whose LDS banks never collide;
 //   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
 //     lanes are 8 classes x 8 sub-ranges of the segment;
 //   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
@@ -318,16 +351,38 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   uint64_t ti = sq / kPiTabStep + 1;
   if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
   const uint32_t np = A.pi_tab[ti];
-  const uint32_t n1 = min(np, A.tier1_end);
+  const uint32_t n0 = min(np, A.tier0_end);
+  const uint32_t n1 = max(n0, min(np, A.tier1_end));
   const uint32_t n2 = max(n1, min(np, A.tier2_end));
   const uint32_t n3 = max(n2, min(np, A.tier3_end));
 
+  {
+    // tier 0: work item = (prime, class pair); half-wave h takes class 2q + h.
+    // The class's K crossings are split into runs of 4m (m odd, 32 runs cover
+    // K), so lane l starts l*m*p dwords after lane 0: with m*p odd, the 32
+    // lanes of a half-wave always address 32 different LDS banks.
+    const uint32_t hl = lane & 31, half = lane >> 5;
+    for (uint32_t it = wave; it < 4 * n0; it += NW) {
+      const uint32_t i = it >> 2;
+      const uint32_t c = ((it & 3) << 1) | half;
+      const uint32_t e = A.primes[i];
+      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+      const uint64_t offl = first_crossing(p, r, c, mod_u64(k0, p), k0, tab);
+      if (offl >= nbytes) continue;
+      const uint32_t j = (uint32_t)offl;
+      const uint32_t K = div_u32(nbytes - 1 - j, p) + 1;  // crossings j + k*p < nbytes
+      const uint32_t m = ((K + 127) >> 7) | 1u;
+      const uint32_t kb = hl * 4 * m;
+      if (kb >= K) continue;
+      cross_run(base, j + kb * p, p, min(4 * m, K - kb), tab[64 + r * 8 + c]);
+    }
+  }
   {
     const uint32_t c = lane & 7, sr = lane >> 3;
     const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
     const uint32_t lo = min(sr * sub_len, nbytes);
     const uint32_t hi = min(lo + sub_len, nbytes);
-    for (uint32_t i = wave; i < n1; i += NW) {
+    for (uint32_t i = n0 + wave; i < n1; i += NW) {
       const uint32_t e = A.primes[i];
       const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
       const uint64_t offl = first_crossing(p, r, c, mod_u64(k0, p), k0, tab);
