@@ -104,7 +104,8 @@ constexpr uint32_t kLdsHist = 0;                                    // kHistBins
 constexpr uint32_t kLdsByteHist = kLdsHist + kHistBins * kHistCopies; // 256 byte values x copies
 constexpr uint32_t kLdsTab = kLdsByteHist + 256 * kHistCopies;        // 3 x 64 bytes = 48 words
 constexpr uint32_t kLdsMisc = kLdsTab + 48;                       // 256 words
-constexpr uint32_t kLdsSeg = kLdsMisc + 256;                      // segment bytes
+constexpr uint32_t kLdsLut = kLdsMisc + 256;                      // 256 x u16 byte-value table = 128 words
+constexpr uint32_t kLdsSeg = kLdsLut + 128;                       // segment bytes
 }  // namespace
 
 uint32_t lds_bytes_for(uint32_t seg_bytes) { return (kLdsSeg * 4) + seg_bytes + 16; }
@@ -152,6 +153,15 @@ __device__ void init_tables(uint32_t* lds) {
     tab[tid] = (uint8_t)(prod / 30);                                     // FRAC
     tab[64 + tid] = (uint8_t)bit;                                        // BITS
     tab[128 + tid] = (uint8_t)((w30_const(c) + 30 - w30_const(r)) % 30); // DM
+  }
+  if (tid < 256) {
+    // per byte value x: W30 of its first and last set bit (bits 0-4, 5-9) and
+    // whether it holds two or more primes (bit 10)
+    const uint32_t x = tid;
+    const uint32_t wf = x ? w30_const(__builtin_ctz(x)) : 0u;
+    const uint32_t wl = x ? w30_const(31 - __builtin_clz(x)) : 0u;
+    reinterpret_cast<uint16_t*>(lds + kLdsLut)[x] =
+        (uint16_t)(wf | (wl << 5) | ((__builtin_popcount(x) >= 2 ? 1u : 0u) << 10));
   }
 }
 
@@ -607,6 +617,7 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   const uint32_t slow = (MODE == kModeStats) ? min(A.large_threshold, 2 * kHistBins) : 0xFFFFFFFFu;
   const uint32_t hbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsHist + copy) * 4;
   const uint32_t bbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsByteHist + copy) * 4;
+  const uint16_t* lut = reinterpret_cast<const uint16_t*>(lds + kLdsLut);
 
   uint32_t carry = 0, wfirst = 0;
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
@@ -639,14 +650,14 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-      const uint32_t pc = __builtin_popcount(x);
+      const uint32_t t = lut[x];
       const uint32_t vbj = 30 * (boff + j);
-      const uint32_t vf = vbj + w30(__builtin_ctz(x | 0x100u) & 7);
-      const uint32_t vl = vbj + w30((31 - __builtin_clz(x | 1u)) & 7);
+      const uint32_t vf = vbj + (t & 31u);
+      const uint32_t vl = vbj + ((t >> 5) & 31u);
       const uint32_t gap = vf - prev;
       const bool ok = (x != 0) & (prev != 0);
       if (ok) lds_add_at(hbase + min(gap >> 1, kHistBins - 1) * (4 * kHistCopies), 1u);
-      if (pc >= 2) lds_add_at(bbase + x * (4 * kHistCopies), 1u);
+      if (t & 0x400u) lds_add_at(bbase + x * (4 * kHistCopies), 1u);
       lm = max(lm, ok ? gap : 0u);
       wfirst = ((x != 0) & (prev == 0)) ? vf : wfirst;
       prev = x ? vl : prev;
