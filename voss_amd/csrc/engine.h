@@ -174,6 +174,10 @@ class Engine {
   };
   DevBuf& ws(int slot, size_t bytes);
   std::vector<DevBuf> ws_;
+  // persistent pinned host staging for downloads, grown on demand
+  void* pinned(size_t bytes);
+  void* pinned_ = nullptr;
+  size_t pinned_bytes_ = 0;
 
   int dev_ = 0, ncu_ = 0;
   std::string name_;

@@ -272,6 +272,18 @@ int Engine::grid_for(uint32_t nseg, const TuneConfig& c) const {
   return (int)std::max<uint64_t>(g, 1);
 }
 
+void* Engine::pinned(size_t bytes) {
+  if (bytes > pinned_bytes_) {
+    if (pinned_) (void)hipHostFree(pinned_);
+    pinned_ = nullptr;
+    pinned_bytes_ = 0;
+    const size_t want = std::max(bytes, (size_t)1 << 20);
+    hip_check(hipHostMalloc(&pinned_, want, hipHostMallocDefault), "hipHostMalloc(staging)");
+    pinned_bytes_ = want;
+  }
+  return pinned_;
+}
+
 DevBuf& Engine::ws(int slot, size_t bytes) {
   // ws_ is sized once in the constructor: references handed out stay valid
   if (slot < 0 || slot >= (int)ws_.size()) throw Error(7, "bad workspace slot");
@@ -455,40 +467,60 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   T.final_out = d_cnt.as<unsigned long long>() + 4;
   hip_check(launch_stitch(T, stream_), "launch(stitch)");
   hip_check(hipEventRecord(e2.e, stream_), "hipEventRecord");
-
-  unsigned int cnts[4];
-  unsigned long long fin[2];
-  unsigned long long tot[3];
-  hip_check(hipMemcpyAsync(cnts, d_cnt.p, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  hip_check(hipMemcpyAsync(fin, d_cnt.as<unsigned long long>() + 4, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  hip_check(hipMemcpyAsync(tot, d_tot.p, 24, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  const uint32_t n_resc = cnts[2];
-  if (n_resc > rescan_cap) throw Error(7, "record rescan list overflow");
-  if (n_resc) {
+  {
+    // records of flagged segments: the item count stays on the device
     RescanArgs RA;
     RA.items = d_resc.as<RescanItem>();
+    RA.n_items = d_cnt.as<unsigned int>() + 2;
+    RA.items_cap = rescan_cap;
     RA.out = d_rout.as<GapRec>();
     RA.n_out = d_cnt.as<unsigned int>() + 3;
     RA.out_cap = rout_cap;
-    hip_check(launch_rescan(cfg_.block, A, RA, (int)n_resc, stream_), "launch(rescan)");
+    hip_check(launch_rescan(cfg_.block, A, RA, grid_for(rescan_cap), stream_), "launch(rescan)");
   }
   hip_check(hipEventRecord(e3.e, stream_), "hipEventRecord");
 
-  std::vector<unsigned long long> dh(kGapBins);
-  std::vector<unsigned long long> bk(std::max<uint64_t>(nbuckets, 1) * 3);
-  hip_check(hipMemcpyAsync(dh.data(), d_hist.p, (size_t)kGapBins * 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  if (nbuckets)
-    hip_check(hipMemcpyAsync(bk.data(), d_bk.p, (size_t)nbuckets * 3 * 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  hip_check(hipMemcpyAsync(cnts, d_cnt.p, 16, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  // one download into pinned staging: counters, totals, histogram, buckets
+  // and the first kHead entries of each list (the rest only if there are more)
+  constexpr uint32_t kHead = 4096;
+  const size_t o_cnt = 0, o_fin = 16, o_tot = 32, o_hist = 64;
+  const size_t o_bk = o_hist + (size_t)kGapBins * 8;
+  const size_t bk_bytes = (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8;
+  const size_t o_lg = (o_bk + bk_bytes + 255) & ~(size_t)255;
+  const size_t o_rc = o_lg + (size_t)kHead * sizeof(GapRec);
+  const size_t o_ro = o_rc + (size_t)kHead * sizeof(GapRec);
+  uint8_t* hs = static_cast<uint8_t*>(pinned(o_ro + (size_t)kHead * sizeof(GapRec)));
+  auto d2h = [&](size_t off, const void* src, size_t n) {
+    hip_check(hipMemcpyAsync(hs + off, src, n, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
+  };
+  d2h(o_cnt, d_cnt.p, 16);
+  d2h(o_fin, d_cnt.as<unsigned long long>() + 4, 16);
+  d2h(o_tot, d_tot.p, 24);
+  d2h(o_hist, d_hist.p, (size_t)kGapBins * 8);
+  if (nbuckets) d2h(o_bk, d_bk.p, (size_t)nbuckets * 3 * 8);
+  d2h(o_lg, d_large.p, (size_t)kHead * sizeof(GapRec));
+  d2h(o_rc, d_rec.p, (size_t)kHead * sizeof(GapRec));
+  d2h(o_ro, d_rout.p, (size_t)kHead * sizeof(GapRec));
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  const unsigned int* cnts = reinterpret_cast<const unsigned int*>(hs + o_cnt);
+  const unsigned long long* fin = reinterpret_cast<const unsigned long long*>(hs + o_fin);
+  const unsigned long long* tot = reinterpret_cast<const unsigned long long*>(hs + o_tot);
+  const unsigned long long* dh = reinterpret_cast<const unsigned long long*>(hs + o_hist);
+  const unsigned long long* bk = reinterpret_cast<const unsigned long long*>(hs + o_bk);
+  if (cnts[2] > rescan_cap) throw Error(7, "record rescan list overflow");
   const uint32_t n_large = std::min(cnts[0], large_cap), n_rec = std::min(cnts[1], rec_cap),
                  n_rout = std::min(cnts[3], rout_cap);
-  std::vector<GapRec> lg(n_large), rc(n_rec), ro(n_rout);
-  if (n_large) hip_check(hipMemcpyAsync(lg.data(), d_large.p, n_large * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  if (n_rec) hip_check(hipMemcpyAsync(rc.data(), d_rec.p, n_rec * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  if (n_rout) hip_check(hipMemcpyAsync(ro.data(), d_rout.p, n_rout * sizeof(GapRec), hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  auto take = [&](size_t off, const DevBuf& src, uint32_t n) {
+    std::vector<GapRec> v(n);
+    const uint32_t head = std::min(n, kHead);
+    if (head) std::memcpy(v.data(), hs + off, (size_t)head * sizeof(GapRec));
+    if (n > head)
+      hip_check(hipMemcpy(v.data() + head, src.as<GapRec>() + head, (size_t)(n - head) * sizeof(GapRec),
+                          hipMemcpyDeviceToHost), "hipMemcpy");
+    return v;
+  };
+  const std::vector<GapRec> lg = take(o_lg, d_large, n_large), rc = take(o_rc, d_rec, n_rec),
+                            ro = take(o_ro, d_rout, n_rout);
 
   times_ = KernelTimes();
   times_.sieve_ms = elapsed(e0, e1);

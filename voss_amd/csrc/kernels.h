@@ -29,6 +29,8 @@ struct StitchArgs {
 
 struct RescanArgs {
   const RescanItem* items;
+  const unsigned int* n_items;  // device-side item count (capped at items_cap)
+  uint32_t items_cap;
   GapRec* out;
   unsigned int* n_out;
   uint32_t out_cap;
@@ -53,7 +55,8 @@ hipError_t prepare_kernels(uint32_t lds_bytes);
 
 hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipStream_t st);
 hipError_t launch_stitch(const StitchArgs& A, hipStream_t st);
-hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st);
+// grid workgroups loop over the *R.n_items flagged segments
+hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int grid, hipStream_t st);
 hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStream_t st);
 // k-th prime (1-based ranks[i]) of sweep segment segs[i] for i < *n_dev; one
 // workgroup per request, n_cap workgroups launched.

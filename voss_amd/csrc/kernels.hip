@@ -1028,7 +1028,9 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   init_tables(lds);
   __syncthreads();
-  const RescanItem item = R.items[blockIdx.x];
+  const uint32_t nitems = min(*R.n_items, R.items_cap);  // same value in every thread
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+  const RescanItem item = R.items[it];
   const uint32_t s = item.seg;
   SegGeom g;
   g.query = 0;
@@ -1122,6 +1124,8 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
     if (lane == 0) excl = 0;
     const uint32_t run = max(stepmax[wave * steps + st], excl);
     if (lm > run) chunk_gaps<true>(q, boff, prev, run, nullptr, R, g.k0, s);
+  }
+  __syncthreads();  // the next item rebuilds the segment in LDS
   }
 }
 
@@ -1355,12 +1359,12 @@ hipError_t launch_stitch(const StitchArgs& A, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int nitems, hipStream_t st) {
-  if (nitems <= 0) return hipSuccess;
+hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int grid, hipStream_t st) {
+  if (grid <= 0) return hipSuccess;
   if (block == 1024)
-    hipLaunchKernelGGL((rescan_kernel<1024>), dim3(nitems), dim3(1024), lds_bytes_for(A.seg_bytes), st, A, R);
+    hipLaunchKernelGGL((rescan_kernel<1024>), dim3(grid), dim3(1024), lds_bytes_for(A.seg_bytes), st, A, R);
   else
-    hipLaunchKernelGGL((rescan_kernel<512>), dim3(nitems), dim3(512), lds_bytes_for(A.seg_bytes), st, A, R);
+    hipLaunchKernelGGL((rescan_kernel<512>), dim3(grid), dim3(512), lds_bytes_for(A.seg_bytes), st, A, R);
   return hipGetLastError();
 }
 
