@@ -172,3 +172,15 @@ def test_invalid_requests(serve):
         assert int(r["value"]) == 0
         r = srv.run_one(serve.request(serve.RANGE, 2, 1))
         assert (int(r["status"]), int(r["value"])) == (0, 0)
+
+
+def test_nth_at_scale(voss, serve, served):
+    # published values (A006988): p(1e11) and p(1e12); answered from the
+    # index a server built, both through the server and the plain entry point
+    known = {10**11: 2_760_727_302_517, 10**12: 29_996_224_275_833}
+    for n, p in known.items():
+        assert voss.stats.nth_prime(n) == p
+    with serve.Server() as srv:
+        for n, p in known.items():
+            r = srv.run_one(serve.request(serve.NTH, n))
+            assert (int(r["status"]), int(r["value"])) == (0, p)

@@ -94,6 +94,14 @@ struct BatchPlan {
 
 using ProgressFn = std::function<void(int seg, int total, uint64_t in_seg, uint64_t cumulative)>;
 
+// Prefix counts of sweep segments (segments of seg_bytes from k = 0):
+// cum[s] = number of primes below 30 * s * seg_bytes, cum[nseg] = pi(hi).
+struct NthIndex {
+  uint64_t hi = 0;
+  uint32_t seg_bytes = 0;
+  std::vector<uint64_t> cum;
+};
+
 class Engine {
  public:
   static Engine& get();  // initialises the device on first use; throws if none
@@ -128,6 +136,9 @@ class Engine {
   void is_prime_batch(const uint64_t* x, size_t n, uint8_t* out);
   void next_prev_batch(const uint64_t* x, size_t n, uint64_t* out, int dir);
   uint64_t nth_prime(uint64_t n);                                         // 1-indexed
+  // Prefix-count index covering [2, hi] for the sweep configuration c; kept
+  // and reused while a later request fits in it (same segment size).
+  std::shared_ptr<const NthIndex> nth_index(uint64_t hi, const TuneConfig& c);
 
   KernelTimes last_times() const { return times_; }
 
@@ -174,6 +185,7 @@ class Engine {
   };
   DevBuf& ws(int slot, size_t bytes);
   std::vector<DevBuf> ws_;
+  std::shared_ptr<const NthIndex> nth_cache_;
   // persistent pinned host staging for downloads, grown on demand
   void* pinned(size_t bytes);
   void* pinned_ = nullptr;

@@ -149,6 +149,24 @@ uint32_t Engine::segment_counts(uint64_t hi, std::vector<uint32_t>& counts, cons
   return nseg;
 }
 
+std::shared_ptr<const NthIndex> Engine::nth_index(uint64_t hi, const TuneConfig& c) {
+  if (nth_cache_ && nth_cache_->hi >= hi && nth_cache_->seg_bytes == c.seg_bytes) return nth_cache_;
+  std::shared_ptr<NthIndex> idx(new NthIndex());
+  std::vector<uint32_t> counts;
+  const uint32_t nseg = segment_counts(hi, counts, c);
+  idx->hi = hi;
+  idx->seg_bytes = c.seg_bytes;
+  idx->cum.resize((size_t)nseg + 1);
+  uint64_t t = 3;  // 2, 3, 5
+  for (uint32_t s = 0; s < nseg; ++s) {
+    idx->cum[s] = t;
+    t += counts[s];
+  }
+  idx->cum[nseg] = t;
+  nth_cache_ = idx;
+  return nth_cache_;
+}
+
 uint64_t Engine::nth_prime(uint64_t n) {
   if (n == 0) throw Error(1, "n must be >= 1 (1-indexed)");
   static const uint64_t small[6] = {2, 3, 5, 7, 11, 13};
@@ -156,15 +174,22 @@ uint64_t Engine::nth_prime(uint64_t n) {
   const double nf = (double)n;
   // Rosser: p_n < n (ln n + ln ln n) for n >= 6
   const uint64_t hi = (uint64_t)(nf * (std::log(nf) + std::log(std::log(nf)))) + 64;
-  std::vector<uint32_t> counts;
-  const uint32_t nseg = segment_counts(hi, counts, cfgs_[0]);
   uint64_t cum = 3;  // 2, 3, 5
   uint32_t s = 0;
-  for (; s < nseg; ++s) {
-    if (cum + counts[s] >= n) break;
-    cum += counts[s];
+  if (nth_cache_ && nth_cache_->hi >= hi && nth_cache_->seg_bytes == cfgs_[0].seg_bytes) {
+    // a resident index (built by a request server) already covers p_n
+    const std::vector<uint64_t>& c = nth_cache_->cum;
+    s = (uint32_t)(std::lower_bound(c.begin(), c.end(), n) - c.begin()) - 1;
+    cum = c[s];
+  } else {
+    std::vector<uint32_t> counts;
+    const uint32_t nseg = segment_counts(hi, counts, cfgs_[0]);
+    for (; s < nseg; ++s) {
+      if (cum + counts[s] >= n) break;
+      cum += counts[s];
+    }
+    if (s == nseg) throw Error(kErrInternal, "internal: could not locate nth prime");
   }
-  if (s == nseg) throw Error(kErrInternal, "internal: could not locate nth prime");
   const uint64_t S = cfgs_[0].seg_bytes;
   const uint64_t lo = std::max<uint64_t>(30 * (uint64_t)s * S, 7);
   const uint64_t top = std::min<uint64_t>(30 * ((uint64_t)s + 1) * S - 1, hi);

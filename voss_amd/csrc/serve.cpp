@@ -80,18 +80,10 @@ Server::Server(const ServeConfig& c) : cfg_(c), eng_(Engine::get()) {
     rcfg_ = eng_.config(1);
     icfg_ = eng_.config(0);
     // nth index: per-segment counts of [7, rosser(nth_max)], prefix-summed
+    // (built once per process and shared by later servers)
     if (cfg_.nth_max >= 4) {
-      index_hi_ = rosser_bound(cfg_.nth_max);
-      std::vector<uint32_t> counts;
-      const uint32_t nseg = eng_.segment_counts(index_hi_, counts, icfg_);
-      cum_.resize((size_t)nseg + 1);
-      uint64_t c = 3;  // 2, 3, 5
-      for (uint32_t s = 0; s < nseg; ++s) {
-        cum_[s] = c;
-        c += counts[s];
-      }
-      cum_[nseg] = c;
-      if (c < cfg_.nth_max) throw Error(VOSS_ERR_INTERNAL, "nth index does not reach nth_max");
+      idx_ = eng_.nth_index(rosser_bound(cfg_.nth_max), icfg_);
+      if (idx_->cum.back() < cfg_.nth_max) throw Error(VOSS_ERR_INTERNAL, "nth index does not reach nth_max");
     }
     if (cfg_.goldbach_max >= 4) {
       uint64_t kb = 0;
@@ -271,9 +263,10 @@ void Server::launch(Slot& s) {
     for (size_t j = 0; j < nT; ++j) {
       const uint64_t n = s.items[s.by_kind[kReqNth][s.nth_dev[j]]].req.a;
       // first segment whose prefix reaches n, minus one: cum[sg] < n <= cum[sg + 1]
-      const size_t sg = (size_t)(std::lower_bound(cum_.begin(), cum_.end(), n) - cum_.begin()) - 1;
+      const std::vector<uint64_t>& cum = idx_->cum;
+      const size_t sg = (size_t)(std::lower_bound(cum.begin(), cum.end(), n) - cum.begin()) - 1;
       segs[j] = (uint32_t)sg;
-      ranks[j] = (uint32_t)(n - cum_[sg]);
+      ranks[j] = (uint32_t)(n - cum[sg]);
     }
     hdr[0] = (uint32_t)nT;
   }
@@ -302,12 +295,13 @@ void Server::launch(Slot& s) {
                 "launch(goldbach)");
     }
     if (nT) {
-      const uint32_t nseg = (uint32_t)(cum_.size() - 1);
+      const uint32_t nseg = (uint32_t)(idx_->cum.size() - 1);
       SieveArgs A = eng_.base_args(nseg, icfg_);
+      A.seg_bytes = idx_->seg_bytes;
       A.k_base = 0;
-      A.k_end = index_hi_ / 30 + 1;
+      A.k_end = idx_->hi / 30 + 1;
       A.lo_val = 7;
-      A.hi_val = index_hi_;
+      A.hi_val = idx_->hi;
       const uint32_t* segs = reinterpret_cast<const uint32_t*>(din + s.in_off[kReqNth]);
       hip_check(launch_select(icfg_.block, A, segs, segs + nT,
                               reinterpret_cast<unsigned long long*>(dout + s.out_off[kReqNth]),

@@ -111,8 +111,7 @@ class Server {
   TuneConfig rcfg_;  // batched-range configuration
   TuneConfig icfg_;  // configuration the nth index was counted with
   // resident state
-  std::vector<uint64_t> cum_;  // cum_[s] = #primes below 30 * s * seg_bytes of the index
-  uint64_t index_hi_ = 0;
+  std::shared_ptr<const NthIndex> idx_;  // nth prefix index (shared with the engine's cache)
   DevBuf gb_bitmap_;           // wheel bitmap of [0, goldbach_max]
   std::vector<std::unique_ptr<Slot>> slots_;
 
