@@ -141,7 +141,11 @@ int voss_amd_server_start(const voss_amd_server_config_t* cfg);
 int voss_amd_server_stop(void);
 /* Enqueue one request (arrival = now); *ticket identifies it for _wait. */
 int voss_amd_server_submit(const voss_amd_request_t* req, uint64_t* ticket);
-/* Block until the ticket's response is ready (each ticket is waited once). */
+/* Block until the ticket's response is ready (each ticket is waited once).
+ * A wait that has begun keeps its server alive until it returns, even if
+ * another thread stops or replaces the server meanwhile (a server answers
+ * every request it accepted before it goes away).  A ticket issued by an
+ * earlier server returns VOSS_ERR_INVALID_ARG. */
 int voss_amd_server_wait(uint64_t ticket, voss_amd_response_t* out);
 /* Open-loop replay: request i is submitted arrival_s[i] seconds after the
  * start; returns every response and the latency/throughput summary. */

@@ -184,3 +184,15 @@ def test_nth_at_scale(voss, serve, served):
         for n, p in known.items():
             r = srv.run_one(serve.request(serve.NTH, n))
             assert (int(r["status"]), int(r["value"])) == (0, p)
+
+
+def test_ticket_of_an_earlier_server(voss, serve):
+    a = serve.Server(nth_max=10**4, goldbach_max=10**4)
+    t = a.submit(serve.request(serve.NEXT, 100))
+    assert int(a.wait(t)["value"]) == 101
+    t2 = a.submit(serve.request(serve.NEXT, 1000))
+    a.close()  # answers t2 before it goes away
+    with serve.Server(nth_max=10**4, goldbach_max=10**4) as b:
+        with pytest.raises(voss.VossError):
+            b.wait(t2)
+        assert int(b.run_one(serve.request(serve.PREV, 100))["value"]) == 97

@@ -731,7 +731,10 @@ extern "C" int voss_amd_synchronize(void) {
 // ====================================================================== serving
 namespace {
 std::mutex g_srv_mu;
-std::unique_ptr<voss::Server> g_srv;
+// shared so that a wait on one thread keeps the server alive while another
+// thread stops or replaces it (its scheduler drains every queued request
+// before the last reference lets it go)
+std::shared_ptr<voss::Server> g_srv;
 
 voss::ServeConfig to_serve_config(const voss_amd_server_config_t* c) {
   voss::ServeConfig s;
@@ -784,7 +787,7 @@ extern "C" int voss_amd_server_start(const voss_amd_server_config_t* cfg) {
   return guarded([&] {
     std::lock_guard<std::mutex> lk(g_srv_mu);
     g_srv.reset();
-    g_srv.reset(new voss::Server(to_serve_config(cfg)));
+    g_srv = std::make_shared<voss::Server>(to_serve_config(cfg));
     return VOSS_OK;
   });
 }
@@ -812,10 +815,10 @@ extern "C" int voss_amd_server_submit(const voss_amd_request_t* req, uint64_t* t
 extern "C" int voss_amd_server_wait(uint64_t ticket, voss_amd_response_t* out) {
   if (!out) { set_err("out pointer is null"); return VOSS_ERR_INVALID_ARG; }
   return guarded([&] {
-    voss::Server* s = nullptr;
+    std::shared_ptr<voss::Server> s;
     {
       std::lock_guard<std::mutex> lk(g_srv_mu);
-      s = g_srv.get();
+      s = g_srv;
     }
     if (!s) throw Error(VOSS_ERR_INVALID_ARG, "server not started");
     to_response(s->wait(ticket), out);

@@ -12,6 +12,7 @@
 #include "serve.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -66,6 +67,8 @@ struct Server::Slot {
 };
 
 Server::Server(const ServeConfig& c) : cfg_(c), eng_(Engine::get()) {
+  static std::atomic<uint64_t> generations{0};
+  gen_ = (++generations) & 0xFFFFFFull;
   if (cfg_.plain) {
     cfg_.slots = 1;
     cfg_.max_batch = 1;
@@ -154,7 +157,7 @@ uint64_t Server::submit(const Request& r, Clock::time_point arrival) {
   it.bytes = bytes_of(r);
   const int v = validate(r);
   std::unique_lock<std::mutex> lk(mu_);
-  it.ticket = next_ticket_++;
+  it.ticket = (gen_ << 40) | next_ticket_++;
   if (v != VOSS_OK) {
     Response& o = done_[it.ticket];
     o.status = v;
@@ -170,6 +173,7 @@ uint64_t Server::submit(const Request& r, Clock::time_point arrival) {
 }
 
 Response Server::wait(uint64_t ticket) {
+  if ((ticket >> 40) != gen_) throw Error(VOSS_ERR_INVALID_ARG, "ticket was issued by another server");
   std::unique_lock<std::mutex> lk(mu_);
   done_cv_.wait(lk, [&] { return done_.count(ticket) != 0; });
   Response r = done_[ticket];

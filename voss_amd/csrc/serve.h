@@ -83,6 +83,7 @@ class Server {
   // Enqueue; arrival is the time latency is measured from.
   uint64_t submit(const Request& r, Clock::time_point arrival);
   // Block until ticket completes; copies its response and forgets it.
+  // Throws for a ticket issued by another server.
   Response wait(uint64_t ticket);
   ServeStats stats();
   const ServeConfig& config() const { return cfg_; }
@@ -120,6 +121,7 @@ class Server {
   std::deque<Item> inbox_;
   std::unordered_map<uint64_t, Response> done_;
   uint64_t next_ticket_ = 1;
+  uint64_t gen_ = 0;  // server generation, in the ticket's top 24 bits
   bool stop_ = false;
   ServeStats st_;
   std::deque<Item> pending_;  // scheduler-owned
