@@ -160,7 +160,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the sieve shards by "
-              "number range with a ~12 KB per-shard stitch, see DESIGN.md", file=sys.stderr)
+              "number range: each GPU sweeps whole segments of its share and the shares are joined by "
+              "the same boundary stitch as segments (about 12 KB per GPU)", file=sys.stderr)
         sys.exit(2)
     N = int(args.n)
 
