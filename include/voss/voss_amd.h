@@ -131,7 +131,10 @@ typedef struct {
     double qps;           /* requests / wall_s */
     double p50_ms, p99_ms, mean_ms, max_ms;
     double warmup_s;      /* building the resident index and bitmap */
-    uint64_t ticks, max_tick, deferred_by_memory, peak_inflight_bytes;
+    uint64_t ticks;               /* device batches launched */
+    uint64_t max_tick;            /* most requests in one batch */
+    uint64_t deferred_by_memory;  /* requests held back at least once by the memory budget */
+    uint64_t peak_inflight_bytes; /* largest working set admitted at once */
 } voss_amd_serve_stats_t;
 
 /* Fill cfg with the defaults. */

@@ -443,7 +443,7 @@ void Server::run() {
       const uint64_t budget = std::min<uint64_t>(cfg_.hbm_budget, dev_room);
       uint64_t bytes = 0;
       while (!pending_.empty() && s.items.size() < cfg_.max_batch) {
-        const Item& it = pending_.front();
+        Item& it = pending_.front();
         if (inflight_bytes_ + bytes + it.bytes > budget) {
           if (s.items.empty() && inflight_bytes_ == 0) {
             // can never fit: fail it rather than wait forever
@@ -457,7 +457,11 @@ void Server::run() {
             pending_.pop_front();
             continue;
           }
-          st_.deferred_by_memory++;
+          if (!it.deferred) {  // count each request once, however long it waits
+            it.deferred = true;
+            std::lock_guard<std::mutex> lk(mu_);
+            st_.deferred_by_memory++;
+          }
           break;
         }
         bytes += it.bytes;

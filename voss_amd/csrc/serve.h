@@ -69,7 +69,7 @@ struct ServeStats {
   double warmup_s = 0;
   uint64_t ticks = 0, requests = 0, max_tick = 0;
   uint64_t peak_inflight_bytes = 0;
-  uint64_t deferred_by_memory = 0;  // admissions postponed because the budget was full
+  uint64_t deferred_by_memory = 0;  // requests held back at least once because the budget was full
 };
 
 class Server {
@@ -97,6 +97,7 @@ class Server {
     Request req;
     Clock::time_point arrival;
     uint64_t bytes;  // device working bytes while in flight
+    bool deferred = false;  // already held back once by the memory budget
   };
   struct Slot;
 
