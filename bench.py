@@ -24,8 +24,10 @@ Two more sections ride on the same line:
          queries/s of the continuous-batching server vs the same kernels run
          one request at a time, plus the saturated (all-at-once) rate of each;
          every response compared between the two.
+  large: full statistics over [2, 1e12], the Goldbach check to 1e9 and the
+         partition counts at multiples of 1e6, against the committed fixture.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1] [--no-batch] [--no-serve]
+    python bench.py [--steps K] [--warmup W] [--n N] [--gpus 1] [--no-batch] [--no-serve] [--no-large]
 """
 
 import argparse
@@ -145,6 +147,41 @@ def bench_serve(n: int):
     return out
 
 
+FIXTURE_C4 = ROOT / "tests" / "golden" / "sweep_1e12.json"
+
+
+def bench_large(st):
+    """The large single-GPU sweep: full statistics over [2, 1e12] (bins 2..1500,
+    mod-4/mod-3 tallies every 1e10), Goldbach existence for every even n in
+    [4, 1e9] and partition counts at n = j * 1e6; exact against the fixture."""
+    import voss_amd
+    f = json.loads(FIXTURE_C4.read_text())
+    N = f["N"]
+    st.full_stats(N, cp_interval=f["checkpoint_interval"], as_numpy=True)  # warm
+    t0 = time.perf_counter()
+    r = st.full_stats(N, cp_interval=f["checkpoint_interval"], as_numpy=True)
+    st.synchronize()
+    sweep_s = time.perf_counter() - t0
+    sweep_dev_ms = st.last_kernel_ms()["total_ms"]
+    hist = {str(i): int(c) for i, c in enumerate(r["hist"][:1501]) if c}
+    exact = (all(r[k] == f[k] for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"))
+             and hist == f["hist_1_1500"] and [list(x) for x in r["records"]] == f["records"]
+             and [list(c) for c in r["checkpoints"]] == f["checkpoints"])
+    t0 = time.perf_counter()
+    g = voss_amd.conjectures.test_goldbach_upto(10**9)
+    gb_s = time.perf_counter() - t0
+    ns = [j * 10**6 for j in range(1, 1001)]
+    t0 = time.perf_counter()
+    counts = st.goldbach_counts(ns)
+    cnt_s = time.perf_counter() - t0
+    return {"workload": "full statistics over [2, 1e12] (tallies every 1e10); Goldbach: every even n in [4, 1e9] "
+                        "has a partition, partition counts at n = j*1e6, j = 1..1000",
+            "sweep_s": sweep_s, "sweep_ints_per_s": N / sweep_s, "sweep_device_ms": sweep_dev_ms,
+            "goldbach_check_s": gb_s, "goldbach_counterexamples": g["counterexamples"],
+            "goldbach_counts_s": cnt_s,
+            "exact_vs_fixture": bool(exact and counts == f["goldbach_counts"] and not g["counterexamples"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +193,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=8000)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -239,6 +277,8 @@ def main():
         line["batch"] = bench_batch(st, reps=5)
     if not args.no_serve:
         line["serve"] = bench_serve(args.serve_requests)
+    if not args.no_large:
+        line["large"] = bench_large(st)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(int(args.cpu_sample), args.cpu_reps)
     print(json.dumps(line))
