@@ -51,15 +51,18 @@ __device__ __forceinline__ void lds_add_at(uint32_t addr, uint32_t v) {
 }
 
 // k mod p for k < 2^52, p < 2^24, from a Newton-refined FP64 reciprocal.
+// k mod p for k < 2^53, 0 < p < 2^24.  The quotient estimate from a refined
+// FP64 reciprocal is within one of the truth, so k - q*p lies in (-p, 2p): an
+// integer far below 2^53, which one fused multiply-add yields exactly.
 __device__ __forceinline__ uint32_t mod_u64(uint64_t k, uint32_t p) {
-  double pd = (double)p;
+  const double pd = (double)p;
   double inv = __builtin_amdgcn_rcp(pd);
-  double e = __builtin_fma(-pd, inv, 1.0);
-  inv = __builtin_fma(inv, e, inv);
-  uint64_t q = (uint64_t)((double)k * inv);
-  int64_t r = (int64_t)(k - q * (uint64_t)p);
-  if (r < 0) r += p;
-  if (r >= (int64_t)p) r -= p;
+  inv = __builtin_fma(inv, __builtin_fma(-pd, inv, 1.0), inv);
+  const double kd = (double)k;
+  const double qd = __builtin_floor(kd * inv);
+  int32_t r = (int32_t)__builtin_fma(-qd, pd, kd);
+  if (r < 0) r += (int32_t)p;
+  if (r >= (int32_t)p) r -= (int32_t)p;
   return (uint32_t)r;
 }
 
