@@ -366,8 +366,12 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   const uint32_t np = A.pi_tab[ti];
   const uint32_t n0 = min(np, A.tier0_end);
   const uint32_t n1 = max(n0, min(np, A.tier1_end));
-  const uint32_t n2 = max(n1, min(np, A.tier2_end));
-  const uint32_t n3 = max(n2, min(np, A.tier3_end));
+  // the sparse tier is exact for every p >= nbytes, so a short segment (a
+  // small range query) moves it down: primes past the pi-table bin that
+  // holds nbytes all exceed nbytes
+  const uint32_t nshort = A.pi_tab[min((nbytes + kPiTabStep - 1) / kPiTabStep, A.pi_tab_len - 1)];
+  const uint32_t n3 = max(n1, min(np, min(A.tier3_end, nshort)));
+  const uint32_t n2 = max(n1, min(n3, A.tier2_end));
 
   {
     // tier 0: work item = (prime, class pair); half-wave h takes class 2q + h.
