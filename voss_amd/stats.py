@@ -20,7 +20,8 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
     tuple (pi, pi_4_1, pi_4_3, pi_3_1, pi_3_2) per x = j * cp_interval),
     large_gaps ((position, gap) with gap >= large_threshold, largest first, at
     most large_cap), records ((position, gap) maximal gaps ascending), n_large,
-    n_records.  as_numpy returns hist as a numpy uint64 array (no copy).
+    n_records.  as_numpy returns hist as a numpy uint64 array and checkpoints
+    as an (n, 5) numpy uint64 array (views, no per-element conversion).
     """
     if not isinstance(N, int):
         raise TypeError("N must be int")
@@ -42,7 +43,8 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
         "pi_3_1": st.pi_3_1, "pi_3_2": st.pi_3_2, "last_prime": st.last_prime,
         "max_gap": st.max_gap, "n_large": st.n_large, "n_records": st.n_records,
         "hist": (_np_view(hist) if as_numpy else list(hist)) if with_hist else None,
-        "checkpoints": [tuple(cps[5 * j + f] for f in range(5)) for j in range(ncp)],
+        "checkpoints": (_np_view(cps)[:5 * ncp].reshape(ncp, 5) if as_numpy
+                        else [tuple(cps[5 * j + f] for f in range(5)) for j in range(ncp)]),
         "large_gaps": [(lg[i].position, lg[i].gap) for i in range(nl)],
         "records": [(rec[i].position, rec[i].gap) for i in range(nr)],
     }
