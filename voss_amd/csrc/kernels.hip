@@ -447,16 +447,19 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       continue;
     }
     const uint32_t q30 = p / 30;
-    const uint32_t* frow = reinterpret_cast<const uint32_t*>(tab + r * 8);
-    const uint32_t* brow = reinterpret_cast<const uint32_t*>(tab + 64 + r * 8);
-    const uint32_t flo = frow[0], fhi = frow[1], blo = brow[0], bhi = brow[1];
-    auto cc = [&](uint32_t c) { return q30 * w30(c) + __builtin_amdgcn_perm(fhi, flo, c | 0x0C0C0C00u); };
-    auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(bhi, blo, c | 0x0C0C0C00u) & 7u; };
-    uint32_t c = 0;  // number of classes with cc_c < rk
-    if (cc(3) < rk) c = 4;
-    if (cc(c + 1) < rk) c += 2;
-    if (cc(c) < rk) c += 1;
-    if (c == 7 && cc(7) < rk) c = 8;
+    const uint2 fr = *reinterpret_cast<const uint2*>(tab + r * 8);
+    const uint2 br = *reinterpret_cast<const uint2*>(tab + 64 + r * 8);
+    auto cc = [&](uint32_t c) { return q30 * w30(c) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
+    auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(br.y, br.x, c | 0x0C0C0C00u) & 7u; };
+    // c = number of classes with cc_c < rk.  cc_c = q30 W[c] + frac_c with
+    // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
+    // counts, every class with W > w does not, and only W == w needs a look.
+    uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
+    if (w * q30 > rk) --w;
+    if ((w + 1) * q30 <= rk) ++w;
+    const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
+    uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
+    if (c < 8 && cc(c) < rk) ++c;
     const uint32_t end = rk + nbytes;
     for (; c < 8; ++c) {
       const uint32_t x = cc(c);
