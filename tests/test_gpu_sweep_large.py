@@ -66,3 +66,22 @@ def test_goldbach_1e9(voss):
 def test_prime_count_1e13(voss):
     # library/STATE.md:102-103: pi(1e13) = 346,065,536,839
     assert voss.primes.prime_count(10**13) == 346_065_536_839
+
+
+def test_full_stats_1e13_known(voss):
+    # published: pi(1e13) (A006880), twin pairs below 1e13 (A007508) and the
+    # last maximal gap below 1e13, 674 after 7,177,162,611,713 (A002386)
+    g = voss.stats.full_stats(10**13, cp_interval=10**12)
+    assert g["prime_count"] == 346_065_536_839
+    assert g["hist"][2] == 15_834_664_872
+    assert g["records"][-1] == (7_177_162_611_713 + 674, 674)
+    assert g["max_gap"] == 674
+    h = g["hist"]
+    assert sum(h) == g["prime_count"] - 1 and sum(i * c for i, c in enumerate(h)) == g["last_prime"] - 2
+    assert g["pi_4_1"] + g["pi_4_3"] + 1 == g["prime_count"] == g["pi_3_1"] + g["pi_3_2"] + 1
+    assert g["checkpoints"][-1][0] == g["prime_count"]
+
+
+def test_prime_count_1e14(voss):
+    # the API's upper limit; pi(1e14) = 3,204,941,750,802 (A006880)
+    assert voss.primes.prime_count(10**14) == 3_204_941_750_802
