@@ -32,7 +32,7 @@ TIER1 = [512, 1024, 2048, 4096, 8192]
 
 
 def _lds_ok(seg, wgs):
-    return (14016 + seg + 16) * wgs <= 160 * 1024  # the kernels keep 14,016 B of tables next to the segment
+    return (10688 + seg + 16) * wgs <= 160 * 1024  # the kernels keep 10,688 B of tables next to the segment
 
 
 def _time(fn, reps):

@@ -18,8 +18,11 @@ constexpr uint32_t kFirstSievingPrime = 23;
 // Primes below this are sieved by tier 0 (a half-wave per prime and class).
 constexpr uint32_t kTier0P = 256;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
-constexpr uint32_t kHistBins = 512;   // LDS gap histogram bins (index gap/2)
-constexpr uint32_t kHistCopies = 4;   // copies, one per 16-lane group
+// LDS gap histogram: bins index gap/2 and cover every gap a batched query
+// bins (2..512); longer gaps fall into the last bin and take an exact slow
+// path.  Copies (one per 8-lane group) keep lanes apart on popular bins.
+constexpr uint32_t kHistBins = 264;
+constexpr uint32_t kHistCopies = 8;
 constexpr uint32_t kGapBins = 50000;  // global histogram: index gap/2 for gap < 100000
 constexpr uint32_t kQueryBins = 257;  // per-query histogram: [i] = gap 2i (i=1..256), [0] = other
 constexpr uint32_t kPiTabStep = 1024; // sqrt table granularity

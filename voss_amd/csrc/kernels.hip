@@ -28,6 +28,12 @@ namespace {
 constexpr uint32_t kW30Lo = 0x0D0B0701u;  // W30[0..3] = 1, 7, 11, 13
 constexpr uint32_t kW30Hi = 0x1D171311u;  // W30[4..7] = 17, 19, 23, 29
 
+// 8 * W30[o] as byte tables for v_perm_b32, in order and reversed.
+constexpr uint32_t kW30x8Lo = 0x68583808u;     // 8, 56, 88, 104
+constexpr uint32_t kW30x8Hi = 0xE8B89888u;     // 136, 152, 184, 232
+constexpr uint32_t kW30x8RevLo = 0x8898B8E8u;  // 232, 184, 152, 136
+constexpr uint32_t kW30x8RevHi = 0x08385868u;  // 104, 88, 56, 8
+
 // W30[o] for o in 0..7 (one v_perm_b32: selector bytes 1..3 = 0x0C give 0).
 __device__ __forceinline__ uint32_t w30(uint32_t o) {
   return __builtin_amdgcn_perm(kW30Hi, kW30Lo, o | 0x0C0C0C00u);
@@ -45,6 +51,10 @@ __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+__device__ __forceinline__ void lds_add(lds_u32* p, uint32_t v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 // LDS atomic add at a byte address (ds_add_u32).
 __device__ __forceinline__ void lds_add_at(uint32_t addr, uint32_t v) {
   __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -104,11 +114,10 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 // ---------------------------------------------------------------- LDS layout
 // u32 word offsets into dynamic LDS.
 constexpr uint32_t kLdsHist = 0;                                    // kHistBins*kHistCopies
-constexpr uint32_t kLdsByteHist = kLdsHist + kHistBins * kHistCopies; // 256 byte values x copies
-constexpr uint32_t kLdsTab = kLdsByteHist + 256 * kHistCopies;        // 3 x 64 bytes = 48 words
+constexpr uint32_t kLdsByteHist = kLdsHist + kHistBins * kHistCopies; // 256 byte values, one copy
+constexpr uint32_t kLdsTab = kLdsByteHist + 256;                      // 3 x 64 bytes = 48 words
 constexpr uint32_t kLdsMisc = kLdsTab + 48;                       // 256 words
-constexpr uint32_t kLdsLut = kLdsMisc + 256;                      // 256 x u16 byte-value table = 128 words
-constexpr uint32_t kLdsSeg = kLdsLut + 128;                       // segment bytes
+constexpr uint32_t kLdsSeg = kLdsMisc + 256;                      // segment bytes
 }  // namespace
 
 uint32_t lds_bytes_for(uint32_t seg_bytes) { return (kLdsSeg * 4) + seg_bytes + 16; }
@@ -156,15 +165,6 @@ __device__ void init_tables(uint32_t* lds) {
     tab[tid] = (uint8_t)(prod / 30);                                     // FRAC
     tab[64 + tid] = (uint8_t)bit;                                        // BITS
     tab[128 + tid] = (uint8_t)((w30_const(c) + 30 - w30_const(r)) % 30); // DM
-  }
-  if (tid < 256) {
-    // per byte value x: W30 of its first and last set bit (bits 0-4, 5-9) and
-    // whether it holds two or more primes (bit 10)
-    const uint32_t x = tid;
-    const uint32_t wf = x ? w30_const(__builtin_ctz(x)) : 0u;
-    const uint32_t wl = x ? w30_const(31 - __builtin_clz(x)) : 0u;
-    reinterpret_cast<uint16_t*>(lds + kLdsLut)[x] =
-        (uint16_t)(wf | (wl << 5) | ((__builtin_popcount(x) >= 2 ? 1u : 0u) << 10));
   }
 }
 
@@ -529,6 +529,17 @@ __device__ __forceinline__ uint32_t chunk_last(uint4 q, uint32_t boff) {
   return 30 * (boff + 4 * wi + (bit >> 3)) + w30(bit & 7);
 }
 
+// Segment-relative value of the lowest set bit of a chunk at byte offset boff (0 if empty).
+__device__ __forceinline__ uint32_t chunk_first(uint4 q, uint32_t boff) {
+  uint32_t w = q.x, wi = 0;
+  if (!w) { w = q.y; wi = 1; }
+  if (!w) { w = q.z; wi = 2; }
+  if (!w) { w = q.w; wi = 3; }
+  if (!w) return 0;
+  const uint32_t bit = __builtin_ctz(w);
+  return 30 * (boff + 4 * wi + (bit >> 3)) + w30(bit & 7);
+}
+
 // Exclusive "last prime before this lane" within one 64-lane step.
 __device__ __forceinline__ uint32_t prev_in_step(uint32_t lastv, uint32_t lane, uint32_t& step_last) {
   uint32_t incl = lastv;
@@ -613,21 +624,33 @@ __device__ void chunk_big_gaps(const SieveArgs& A, uint32_t* lds, uint4 q, uint3
 // prime goes to the LDS gap histogram; the gaps between primes of the same
 // byte depend only on the byte value, so a byte with >= 2 primes bumps a
 // histogram of byte values instead, converted to gaps when flushed.
+//
+// Positions inside the byte loop are 8 * (segment-relative value): a gap g
+// then is 8g, half the byte offset 32 * (g / 2) of its histogram bin (eight
+// u32 copies per bin).  The wheel offsets of a byte's first and last prime
+// come from v_perm byte tables (no LDS read).  A lane with no earlier prime in its wave starts from
+// its chunk's first byte; the one spurious gap that gives (chunk start to the
+// first prime) is taken back after the loop.  Zero bytes still raise the
+// chunk's running maximum `lm` by their distance from the last prime; that
+// never exceeds the gap that closes over them, so `lm` stays a bound.
 template <int MODE, int BS>
 __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
-  const uint32_t copy = lane >> 4;
+  const uint32_t copy = lane >> 3;
   const uint32_t wave_bytes = A.seg_bytes / NW;
   const uint32_t M41 = (g.k0 & 1) ? 0x99669966u : 0x66996699u;
   const uint32_t M31 = 0x2B2B2B2Bu;
   const bool partial = g.vlo > 0 || g.vhi < 8 * g.nbytes;
   // gaps at or above this take the exact slow path (stats mode only)
   const uint32_t slow = (MODE == kModeStats) ? min(A.large_threshold, 2 * kHistBins) : 0xFFFFFFFFu;
-  const uint32_t hbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsHist + copy) * 4;
-  const uint32_t bbase = seg_base_addr(lds) - kLdsSeg * 4 + (kLdsByteHist + copy) * 4;
-  const uint16_t* lut = reinterpret_cast<const uint16_t*>(lds + kLdsLut);
+  // The kernels declare no static LDS (prepare_kernels checks), so the
+  // dynamic array starts at LDS address 0: 4x addresses byte value x in the
+  // byte-value histogram through the instruction offset field.
+  lds_u8* const H8 = (lds_u8*)(lds_u32*)lds + (kLdsHist + copy) * 4;
+  constexpr uint32_t kCap8 = (kHistBins - 1) * 16;  // last bin collects longer gaps
+  static_assert(kHistCopies == 8, "bin stride 32 bytes = 2 * (8 * gap)");
 
   uint32_t carry = 0, wfirst = 0;
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
@@ -655,23 +678,31 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     uint32_t step_last;
     const uint32_t prev0 = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
     carry = max(carry, step_last);
-    uint32_t prev = prev0, lm = 0;
+    const uint32_t vb = 240u * boff;
+    uint32_t prev = prev0 ? 8u * prev0 : vb, lm = 0;
     const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-      const uint32_t t = lut[x];
-      const uint32_t vbj = 30 * (boff + j);
-      const uint32_t vf = vbj + (t & 31u);
-      const uint32_t vl = vbj + ((t >> 5) & 31u);
-      const uint32_t gap = vf - prev;
-      const bool ok = (x != 0) & (prev != 0);
-      if (ok) lds_add_at(hbase + min(gap >> 1, kHistBins - 1) * (4 * kHistCopies), 1u);
-      if (t & 0x400u) lds_add_at(bbase + x * (4 * kHistCopies), 1u);
-      lm = max(lm, ok ? gap : 0u);
-      wfirst = ((x != 0) & (prev == 0)) ? vf : wfirst;
-      prev = x ? vl : prev;
+      // 8 * W30 of the first and last set bit from two byte tables (v_perm):
+      // selector 8 (x = 0) reads 0x00 from the sign of table byte 1 (56)
+      const uint32_t wf8 = __builtin_amdgcn_perm(kW30x8Hi, kW30x8Lo, __builtin_ctz(x | 0x100u)) & 0xFFu;
+      const uint32_t wl8 = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, __builtin_clz(x << 24)) & 0xFFu;
+      const uint32_t bj = vb + 240u * j;
+      const uint32_t gap8 = bj + wf8 - prev;
+      lm = max(lm, gap8);
+      if (x) {
+        lds_add((lds_u32*)(H8 + 2 * min(gap8, kCap8)), 1u);
+        prev = bj + wl8;
+      }
+      if (__builtin_popcount(x) > 1) lds_add((lds_u32*)((lds_u8*)(uintptr_t)(4 * x) + kLdsByteHist * 4), 1u);
     }
+    if (prev0 == 0 && (q.x | q.y | q.z | q.w) != 0) {
+      const uint32_t vfirst = chunk_first(q, boff);
+      lds_add((lds_u32*)(H8 + 2 * min(8u * vfirst - vb, kCap8)), 0xFFFFFFFFu);
+      wfirst = vfirst;
+    }
+    lm >>= 3;
     // internal gaps are at most 28; count them in the maximum only coarsely
     // (an over-estimate only triggers a records rescan of tiny segments)
     maxg = max(maxg, lm);
@@ -706,12 +737,8 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
 template <int BS, typename Sink>
 __device__ void flush_histograms(uint32_t* lds, Sink sink) {
   for (uint32_t x = threadIdx.x; x < 256; x += BS) {
-    uint32_t s = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < kHistCopies; ++c) {
-      s += lds[kLdsByteHist + x * kHistCopies + c];
-      lds[kLdsByteHist + x * kHistCopies + c] = 0;
-    }
+    const uint32_t s = lds[kLdsByteHist + x];
+    lds[kLdsByteHist + x] = 0;
     if (!s) continue;
     for (uint32_t nib = internal_gaps(x); nib; nib >>= 4)
       lds_add(lds + kLdsHist + (nib & 15) * kHistCopies + (x & (kHistCopies - 1)), s);
@@ -771,7 +798,7 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
   extern __shared__ uint32_t lds[];
   constexpr int NW = BS / 64;
   init_tables(lds);
-  for (uint32_t i = threadIdx.x; i < (kHistBins + 256) * kHistCopies; i += BS) lds[kLdsHist + i] = 0;
+  for (uint32_t i = threadIdx.x; i < kHistBins * kHistCopies + 256; i += BS) lds[kLdsHist + i] = 0;
   uint32_t cur_query = 0xFFFFFFFFu;
   uint32_t since_flush = 0;
   __syncthreads();
@@ -1360,6 +1387,11 @@ hipError_t prepare_kernels(uint32_t lds_bytes) {
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
+    // the walk addresses LDS tables absolutely: dynamic LDS must start at 0
+    hipFuncAttributes fa;
+    e = hipFuncGetAttributes(&fa, f);
+    if (e != hipSuccess) return e;
+    if (fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
   }
   return hipSuccess;
 }
