@@ -312,7 +312,9 @@ __device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t 
 
 // Every crossing of p (rk = k0 mod p) in the segment, class by class, from
 // two packed 64-bit LDS rows: off = min(cc + p - rk, cc - rk), the p*p rule
-// applied where it can bite.  Suits primes with a few crossings per class.
+// applied where it can bite.  LONG walks each class with the unrolled
+// cross_class loop (many crossings per class), otherwise one at a time.
+template <bool LONG = false>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
                                                     uint32_t nbytes, const uint8_t* tab) {
   const uint32_t q30 = p / 30;
@@ -330,9 +332,13 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
       if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
     }
     const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
-    while (off < nbytes) {
-      cross_one(base, off, bit);
-      off += p;
+    if (LONG) {
+      if (off < nbytes) cross_class(base, off, p, nbytes, bit);
+    } else {
+      while (off < nbytes) {
+        cross_one(base, off, bit);
+        off += p;
+      }
     }
   }
 }
@@ -412,22 +418,23 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (j < hi) cross_class(base, j, p, hi, tab[64 + r * 8 + c]);
     }
   }
+  // Tiers 2-4 load the next turn's prime before working on this one, so the
+  // table read's latency overlaps the crossings.
+  uint32_t e_next = (n1 + 64 * wave + lane < n2) ? A.primes[n1 + 64 * wave + lane] : 0u;
   for (uint32_t b = n1 + 64 * wave; b < n2; b += 64 * NW) {
     const uint32_t i = b + lane;
+    const uint32_t e = e_next;
+    e_next = (i + 64 * NW < n2) ? A.primes[i + 64 * NW] : 0u;
     if (i >= n2) continue;
-    const uint32_t e = A.primes[i];
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-    const uint32_t rk = mod_u64(k0, p);
-#pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) {
-      const uint64_t offl = first_crossing(p, r, c, rk, k0, tab);
-      if (offl < nbytes) cross_class(base, (uint32_t)offl, p, nbytes, tab[64 + r * 8 + c]);
-    }
+    cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab);
   }
+  e_next = (n2 + 64 * wave + lane < n3) ? A.primes[n2 + 64 * wave + lane] : 0u;
   for (uint32_t b = n2 + 64 * wave; b < n3; b += 64 * NW) {
     const uint32_t i = b + lane;
+    const uint32_t e = e_next;
+    e_next = (i + 64 * NW < n3) ? A.primes[i + 64 * NW] : 0u;
     if (i >= n3) continue;
-    const uint32_t e = A.primes[i];
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab);
   }
@@ -436,10 +443,12 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   // so the classes that hit the window [rk, rk + nbytes) mod p form one run
   // (two when the window wraps past p): a three-step search finds its start
   // and only the run is visited.
+  e_next = (n3 + 64 * wave + lane < np) ? A.primes[n3 + 64 * wave + lane] : 0u;
   for (uint32_t b = n3 + 64 * wave; b < np; b += 64 * NW) {
     const uint32_t i = b + lane;
+    const uint32_t e = e_next;
+    e_next = (i + 64 * NW < np) ? A.primes[i + 64 * NW] : 0u;
     if (i >= np) continue;
-    const uint32_t e = A.primes[i];
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     const uint32_t rk = mod_u64(k0, p);
     if (30 * k0 < (uint64_t)p * p + 30ull * p) {  // the p*p rule can bite: general path
