@@ -316,15 +316,15 @@ __device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t 
 // cross_class loop (many crossings per class), otherwise one at a time.
 template <bool LONG = false>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
-                                                    uint32_t nbytes, const uint8_t* tab) {
+                                                    uint32_t nbytes, const uint8_t* tab, uint32_t plow) {
   const uint32_t q30 = p / 30;
   const uint32_t prk = p - rk;
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
-  const bool low = 30 * k0 < (uint64_t)p * p + 30ull * p;  // the p*p rule can bite
+  const bool low = p >= plow;  // the p*p rule can bite
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const uint32_t cc = q30 * w30_const(c) + (uint32_t)((frac >> (8 * c)) & 0xFF);
+    const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
     uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
     if (low) {
       const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
@@ -378,6 +378,16 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   const uint32_t nshort = A.pi_tab[min((nbytes + kPiTabStep - 1) / kPiTabStep, A.pi_tab_len - 1)];
   const uint32_t n3 = max(n1, min(np, min(A.tier3_end, nshort)));
   const uint32_t n2 = max(n1, min(n3, A.tier2_end));
+  // the p*p rule can bite exactly for p >= plow, the least p with
+  // p * (p + 30) > 30 * k0 (one test per prime instead of a 64-bit product)
+  uint32_t plow;
+  {
+    const uint64_t t30 = 30 * k0;
+    uint64_t x = (uint64_t)((__builtin_sqrt(900.0 + 4.0 * (double)t30) - 30.0) * 0.5);
+    while (x > 0 && (x - 1) * (x + 29) > t30) --x;
+    while (x * (x + 30) <= t30) ++x;
+    plow = (uint32_t)min(x, (uint64_t)0xFFFFFFFFu);
+  }
 
   {
     // tier 0: work item = (prime, class pair); half-wave h takes class 2q + h.
@@ -427,7 +437,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     e_next = (i + 64 * NW < n2) ? A.primes[i + 64 * NW] : 0u;
     if (i >= n2) continue;
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-    cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab);
+    cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
   }
   e_next = (n2 + 64 * wave + lane < n3) ? A.primes[n2 + 64 * wave + lane] : 0u;
   for (uint32_t b = n2 + 64 * wave; b < n3; b += 64 * NW) {
@@ -436,7 +446,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     e_next = (i + 64 * NW < n3) ? A.primes[i + 64 * NW] : 0u;
     if (i >= n3) continue;
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-    cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab);
+    cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
   }
   // tier 4 (p >= seg_bytes): at most one crossing per class.  The classes'
   // first multiples cc_c (byte offsets within one period p) increase with c,
@@ -451,21 +461,21 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     if (i >= np) continue;
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     const uint32_t rk = mod_u64(k0, p);
-    if (30 * k0 < (uint64_t)p * p + 30ull * p) {  // the p*p rule can bite: general path
-      cross_prime_classes(base, p, r, rk, k0, nbytes, tab);
+    if (p >= plow) {  // the p*p rule can bite: general path
+      cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
       continue;
     }
     const uint32_t q30 = p / 30;
     const uint2 fr = *reinterpret_cast<const uint2*>(tab + r * 8);
     const uint2 br = *reinterpret_cast<const uint2*>(tab + 64 + r * 8);
-    auto cc = [&](uint32_t c) { return q30 * w30(c) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
+    auto cc = [&](uint32_t c) { return __umul24(q30, w30(c)) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
     auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(br.y, br.x, c | 0x0C0C0C00u) & 7u; };
     // c = number of classes with cc_c < rk.  cc_c = q30 W[c] + frac_c with
     // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
     // counts, every class with W > w does not, and only W == w needs a look.
     uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
-    if (w * q30 > rk) --w;
-    if ((w + 1) * q30 <= rk) ++w;
+    if (__umul24(w, q30) > rk) --w;
+    if (__umul24(w + 1, q30) <= rk) ++w;
     const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
     uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
     if (c < 8 && cc(c) < rk) ++c;
