@@ -101,6 +101,7 @@ Engine::Engine() {
 
   load_tune_file();
   tier0_p_ = env_u32("VOSS_TIER0_P", kTier0P);
+  tier2_div_ = std::max<uint32_t>(1, env_u32("VOSS_TIER2_DIV", 4));
   if (std::getenv("VOSS_SEG_BYTES") || std::getenv("VOSS_BLOCK") || std::getenv("VOSS_WGS_PER_CU") ||
       std::getenv("VOSS_TIER1_P")) {
     TuneConfig c = cfgs_[0];
@@ -260,7 +261,7 @@ SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   A.nseg = nseg;
   A.tier0_end = tier_index(tier0_p_);
   A.tier1_end = tier_index(c.tier1_p);
-  A.tier2_end = tier_index(c.seg_bytes / 4);
+  A.tier2_end = tier_index(c.seg_bytes / tier2_div_);
   A.tier3_end = tier_index(c.seg_bytes);
   A.large_threshold = 0xFFFFFFFFu;
   return A;
