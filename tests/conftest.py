@@ -26,6 +26,6 @@ def oracle():
 def voss():
     so = ROOT / "voss_amd" / "libvoss.so"
     if not so.exists():
-        pytest.fail("libvoss.so is not built: run `python -m voss_amd.build` first")
+        pytest.fail("libvoss.so is not built: run `python voss_amd/build.py` first")
     import voss_amd
     return voss_amd

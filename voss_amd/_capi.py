@@ -38,7 +38,7 @@ def _find_library() -> str:
     for cand in (here / "libvoss.so", here.parent / "library" / "build" / "libvoss.so"):
         if cand.exists():
             return str(cand)
-    raise VossError("libvoss.so not found: build it with `python -m voss_amd.build` "
+    raise VossError("libvoss.so not found: build it with `python voss_amd/build.py` "
                     "or set VOSS_LIBRARY_PATH")
 
 

@@ -1,6 +1,8 @@
 """Build libvoss.so (host C++ engine + gfx950 HIP kernels) in-tree.
 
-    python -m voss_amd.build          # or voss_amd.build.build()
+    python voss_amd/build.py
+
+(run as a script: importing the package itself needs the built library)
 
 Every translation unit is compiled by hipcc for gfx950 into _build/, then
 linked into voss_amd/libvoss.so next to this file (where the Python
