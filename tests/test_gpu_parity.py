@@ -87,6 +87,41 @@ def test_statistics_match_oracle_moments(voss, oracle):
     assert abs(s.std_dev - math.sqrt(var)) < 1e-9
 
 
+def test_export_csv(voss, oracle, tmp_path):
+    # library/tests/test_primes.py:490-519 (three files, stats header, total_gaps
+    # line) plus every line's content against the oracle (io/csv.cpp:54-110 formats)
+    o = oracle.full_stats(10**6)
+    prefix = str(tmp_path / "voss_test")
+    with voss.primes.Context(10**6) as c:
+        files = c.export_csv(prefix)
+        s = c.statistics()
+    assert files == [prefix + "_chebyshev.csv", prefix + "_large_gaps.csv", prefix + "_stats.csv"]
+    cheb = open(files[0]).read().splitlines()
+    assert cheb == ["pi_4_1,pi_4_3,difference", f"{o['pi_4_1']},{o['pi_4_3']},{o['pi_4_3'] - o['pi_4_1']}"]
+    assert cheb[1] == "39175,39322,147"
+    assert open(files[1]).read().splitlines() == ["position,gap"]
+    stats = open(files[2]).read().splitlines()
+    assert stats[0] == "metric,value" and "total_gaps,78497" in stats
+    assert stats[1:5] == [f"mean_gap,{s.mean_gap:.6f}", f"std_dev,{s.std_dev:.6f}", f"skewness,{s.skewness:.6f}",
+                          f"kurtosis,{s.kurtosis:.6f}"]
+    assert stats[1] == "mean_gap,12.739098"  # (p_last - 2) / (pi - 1) = 999981 / 78497 at 1e6
+
+
+def test_export_csv_large_gaps_1e12(voss, tmp_path):
+    # the 11 gaps >= 500 below 1e12 (library/tests/test_primes.py:447-458), in
+    # the list's order: gap descending, then position ascending
+    prefix = str(tmp_path / "big")
+    with voss.primes.Context(10**12) as c:
+        c.export_csv(prefix)
+        gaps = c.large_gaps()
+    rows = open(prefix + "_large_gaps.csv").read().splitlines()
+    assert rows[0] == "position,gap" and len(rows) == 12
+    assert rows[1] == "738832928467,540"
+    assert rows[1:] == [f"{g.position},{g.gap}" for g in gaps]
+    parsed = [tuple(map(int, r.split(","))) for r in rows[1:]]
+    assert parsed == sorted(parsed, key=lambda t: (-t[1], t[0]))
+
+
 @pytest.mark.parametrize("a,b", [(2, 10), (10, 30), (100, 130), (10, 100), (0, 1), (2, 2), (90, 110), (2, 1000),
                                  (10**9, 10**9 + 10**4), (999_999_000, 1_000_001_000), (7, 7), (8, 10),
                                  (3932150, 3932170)])

@@ -874,56 +874,78 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
     walk_segment<MODE, BS>(A, lds, g, split);
     __syncthreads();
 
-    if (threadIdx.x == 0) {
-      const uint32_t* m = lds + kLdsMisc + kMiscWave;
-      uint32_t sfirst = 0, slast = 0, smax = 0, scnt = 0, s41 = 0, s31 = 0, lc = 0, l41 = 0, l31 = 0;
-      for (int w = 0; w < NW; ++w) {
-        const uint32_t* mw = m + 8 * w;
-        scnt += mw[2];
-        if (MODE == kModeStats) { s41 += mw[3]; s31 += mw[4]; lc += mw[6]; l41 += mw[7]; l31 += m[8 * NW + w]; }
-        if (MODE != kModeCount && mw[2]) {
-          smax = max(smax, mw[5]);
-          if (slast) {
-            uint32_t gap = mw[0] - slast;
-            uint32_t dummy = 0;
-            note_gap<MODE>(A, lds, gap, mw[0], g, 0, dummy);
-            smax = max(smax, gap);
-          } else {
-            sfirst = mw[0];
-          }
-          slast = mw[1];
+    if (threadIdx.x < 64) {
+      // Wave 0 joins the per-wave records, lane w holding wave w's: the
+      // waves' last primes increase with w, so a max-scan gives each wave
+      // the last prime before it; sums and maxima are 16-lane reductions
+      // (lanes >= NW hold zeros) and the global adds go out one per lane.
+      const uint32_t lane = threadIdx.x;
+      const uint32_t* mw = lds + kLdsMisc + kMiscWave + 8 * (lane < NW ? lane : 0);
+      const bool mine = lane < NW;
+      const uint32_t wc = mine ? mw[2] : 0u;
+      const bool has = wc != 0;
+      const uint32_t wf = has ? mw[0] : 0u, wl = has ? mw[1] : 0u;
+      uint32_t incl = wl;
+#pragma unroll
+      for (int d = 1; d < NW; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= (uint32_t)d) incl = max(incl, t);
+      }
+      uint32_t before = (uint32_t)__shfl_up((int)incl, 1, 64);
+      if (lane == 0) before = 0;
+      uint32_t smax = 0;
+      if (MODE != kModeCount && has) {
+        smax = mw[5];
+        if (before) note_gap<MODE>(A, lds, wf - before, wf, g, 0, smax);
+      }
+      const uint64_t hb = __ballot(has);
+      const uint32_t sfirst = (uint32_t)__shfl((int)wf, hb ? (int)__builtin_ctzll(hb) : 0, 64);
+      const uint32_t slast = (uint32_t)__shfl((int)incl, NW - 1, 64);
+      auto sum16 = [](uint32_t v) {
+#pragma unroll
+        for (int d = 8; d >= 1; d >>= 1) v += (uint32_t)__shfl_xor((int)v, d, 16);
+        return v;
+      };
+      const uint32_t scnt = sum16(wc);
+      if (MODE != kModeCount) {
+#pragma unroll
+        for (int d = 8; d >= 1; d >>= 1) smax = max(smax, (uint32_t)__shfl_xor((int)smax, d, 16));
+      }
+      if (lane == 0) {
+        if (A.summaries) {
+          SegSummary ss;
+          ss.first = sfirst; ss.last = slast; ss.maxgap = smax; ss.count = scnt;
+          A.summaries[(MODE == kModeBatch) ? s : A.seg_base + s] = ss;
         }
-      }
-      if (A.summaries) {
-        SegSummary ss;
-        ss.first = sfirst; ss.last = slast; ss.maxgap = smax; ss.count = scnt;
-        A.summaries[(MODE == kModeBatch) ? s : A.seg_base + s] = ss;
-      }
-      if (MODE == kModeBatch) {
-        atomicAdd(A.qcount + g.query, (unsigned long long)scnt);
-      } else {
-        atomicAdd(A.totals + 0, (unsigned long long)scnt);
+        if (MODE == kModeBatch) atomicAdd(A.qcount + g.query, (unsigned long long)scnt);
+        if (MODE == kModeCount) atomicAdd(A.totals + 0, (unsigned long long)scnt);
       }
       if (MODE == kModeStats) {
-        atomicAdd(A.totals + 1, (unsigned long long)s41);
-        atomicAdd(A.totals + 2, (unsigned long long)s31);
-        if (A.cp_interval && b0 < A.nbuckets) {
-          unsigned long long* bk = A.buckets;
-          if (split == 0xFFFFFFFFu) {
-            atomicAdd(bk + 3 * b0 + 0, (unsigned long long)scnt);
-            atomicAdd(bk + 3 * b0 + 1, (unsigned long long)s41);
-            atomicAdd(bk + 3 * b0 + 2, (unsigned long long)s31);
-          } else {
-            atomicAdd(bk + 3 * b0 + 0, (unsigned long long)lc);
-            atomicAdd(bk + 3 * b0 + 1, (unsigned long long)l41);
-            atomicAdd(bk + 3 * b0 + 2, (unsigned long long)l31);
-            if (b1 < A.nbuckets) {
-              atomicAdd(bk + 3 * b1 + 0, (unsigned long long)(scnt - lc));
-              atomicAdd(bk + 3 * b1 + 1, (unsigned long long)(s41 - l41));
-              atomicAdd(bk + 3 * b1 + 2, (unsigned long long)(s31 - l31));
-            }
+        const uint32_t s41 = sum16(mine ? mw[3] : 0u), s31 = sum16(mine ? mw[4] : 0u);
+        const uint32_t lc = sum16(mine ? mw[6] : 0u), l41 = sum16(mine ? mw[7] : 0u);
+        const uint32_t l31 = sum16(mine ? lds[kLdsMisc + kMiscWave + 8 * NW + lane] : 0u);
+        // lanes 0-2: totals; 3-5: the first checkpoint bucket; 6-8: the
+        // second (segments split by a checkpoint)
+        const uint32_t k = lane % 3;
+        const uint32_t vf = k == 0 ? scnt : (k == 1 ? s41 : s31);
+        const uint32_t vl = k == 0 ? lc : (k == 1 ? l41 : l31);
+        unsigned long long* dst = nullptr;
+        uint32_t val = 0;
+        if (lane < 3) {
+          dst = A.totals + k;
+          val = vf;
+        } else if (lane < 6) {
+          if (A.cp_interval && b0 < A.nbuckets) {
+            dst = A.buckets + 3 * b0 + k;
+            val = split == 0xFFFFFFFFu ? vf : vl;
+          }
+        } else if (lane < 9) {
+          if (A.cp_interval && split != 0xFFFFFFFFu && b1 < A.nbuckets) {
+            dst = A.buckets + 3 * b1 + k;
+            val = vf - vl;
           }
         }
+        if (dst) atomicAdd(dst, (unsigned long long)val);
       }
     }
     __syncthreads();
