@@ -126,6 +126,7 @@ namespace {
 
 // misc word layout
 constexpr uint32_t kMiscItem = 0;       // current work item
+constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work item of the segment
 constexpr uint32_t kMiscWave = 8;       // 8 words per wave below
 // per wave: [0] first, [1] last, [2] count, [3] c41, [4] c31, [5] maxgap,
 //           [6] lower count, [7] lower c41 ... (lower c31 in [kMiscWave + 8*NW])
@@ -179,6 +180,7 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
 #pragma unroll
   for (int k = 0; k < kPresieveGroups; ++k) off[k] = (uint32_t)(g.k0 % kPresievePeriods[k]);
   const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
+  if (threadIdx.x == 0) lds[kLdsMisc + kMiscSieveCtr] = 0;  // sieve_segment's work counter
   for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
     uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
@@ -389,77 +391,96 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     plow = (uint32_t)min(x, (uint64_t)0xFFFFFFFFu);
   }
 
-  {
-    // tier 0: work item = (prime, class pair); half-wave h takes class 2q + h.
-    // The class's K crossings are split into runs of 4m (m odd, 32 runs cover
-    // K), so lane l starts l*m*p dwords after lane 0: with m*p odd, the 32
-    // lanes of a half-wave always address 32 different LDS banks.
-    const uint32_t hl = lane & 31, half = lane >> 5;
-    for (uint32_t it = wave; it < 4 * n0; it += NW) {
-      const uint32_t i = it >> 2;
-      const uint32_t c = ((it & 3) << 1) | half;
-      const uint32_t e = A.primes[i];
-      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-      const uint64_t offl = first_crossing(p, r, c, mod_u64(k0, p), k0, tab);
-      if (offl >= nbytes) continue;
-      const uint32_t j = (uint32_t)offl;
-      const uint32_t K = div_u32(nbytes - 1 - j, p) + 1;  // crossings j + k*p < nbytes
-      const uint32_t m = ((K + 127) >> 7) | 1u;
-      const uint32_t kb = hl * 4 * m;
-      if (kb >= K) continue;
-      cross_run(base, j + kb * p, p, min(4 * m, K - kb), tab[64 + r * 8 + c]);
-    }
-  }
-  {
-    const uint32_t c = lane & 7, sr = lane >> 3;
-    const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
-    const uint32_t lo = min(sr * sub_len, nbytes);
-    const uint32_t hi = min(lo + sub_len, nbytes);
-    for (uint32_t i = n0 + wave; i < n1; i += NW) {
-      const uint32_t e = A.primes[i];
-      const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-      const uint64_t offl = first_crossing(p, r, c, mod_u64(k0, p), k0, tab);
-      if (offl >= hi) continue;
-      uint32_t j = (uint32_t)offl;
-      if (j < lo) {
-        const uint32_t back = mod_u32(lo - j, p);
-        j = lo + (back ? p - back : 0u);
+  // Work items of tiers 0-3, heaviest first: the 64-prime chunks of tier 2
+  // (its first chunk alone is a large share of a wave's work), tier-0
+  // (prime, class pair) items, tier-1 primes, then the 64-prime chunks of
+  // tier 3.  Each wave takes its next item from an LDS counter (cleared by
+  // init_segment), so waves that drew heavy items take fewer and all of them
+  // reach tier 4 together.  Tier 4's chunks cost about the same each (and a
+  // short segment has thousands of them) and are dealt round-robin.
+  const uint32_t e2 = (n2 - n1 + 63) >> 6;
+  const uint32_t e0 = e2 + 4 * n0;
+  const uint32_t e1 = e0 + (n1 - n0);
+  const uint32_t e3 = e1 + ((n3 - n2 + 63) >> 6);
+  auto take = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0)
+      v = __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((kLdsMisc + kMiscSieveCtr) * 4), 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __builtin_amdgcn_readlane(v, 0);
+  };
+  // this lane's table entry for a chunk item (0: none); loaded one item
+  // ahead so the read's latency overlaps the current item's crossings
+  auto lane_entry = [&](uint32_t it) -> uint32_t {
+    uint32_t i = 0xFFFFFFFFu, end = 0;
+    if (it < e2) { i = n1 + 64 * it + lane; end = n2; }
+    else if (it >= e1 && it < e3) { i = n2 + 64 * (it - e1) + lane; end = n3; }
+    return i < end ? A.primes[i] : 0u;
+  };
+  uint32_t it = take();
+  uint32_t e = lane_entry(it);
+  while (it < e3) {
+    const uint32_t nit = take();
+    const uint32_t ne = lane_entry(nit);
+    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+    if (it < e2) {
+      // tier 2: one prime per lane, each class walked by the unrolled loop
+      if (e) cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
+    } else if (it < e0) {
+      // tier 0: item = (prime, class pair); half-wave h takes class 2q + h.
+      // The class's K crossings are split into runs of 4m (m odd, 32 runs
+      // cover K), so lane l starts l*m*p dwords after lane 0: with m*p odd,
+      // the 32 lanes of a half-wave always address 32 different LDS banks.
+      const uint32_t t0 = it - e2;
+      const uint32_t hl = lane & 31, half = lane >> 5;
+      const uint32_t c = ((t0 & 3) << 1) | half;
+      const uint32_t e0i = A.primes[t0 >> 2];
+      const uint32_t p0 = e0i & 0xFFFFFFu, r0 = e0i >> 24;
+      const uint64_t offl = first_crossing(p0, r0, c, mod_u64(k0, p0), k0, tab);
+      if (offl < nbytes) {
+        const uint32_t j = (uint32_t)offl;
+        const uint32_t K = div_u32(nbytes - 1 - j, p0) + 1;  // crossings j + k*p < nbytes
+        const uint32_t m = ((K + 127) >> 7) | 1u;
+        const uint32_t kb = hl * 4 * m;
+        if (kb < K) cross_run(base, j + kb * p0, p0, min(4 * m, K - kb), tab[64 + r0 * 8 + c]);
       }
-      if (j < hi) cross_class(base, j, p, hi, tab[64 + r * 8 + c]);
+    } else if (it < e1) {
+      // tier 1: one prime per wave, lanes = 8 classes x 8 sub-ranges
+      const uint32_t c = lane & 7, sr = lane >> 3;
+      const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
+      const uint32_t lo = min(sr * sub_len, nbytes);
+      const uint32_t hi = min(lo + sub_len, nbytes);
+      const uint32_t e1i = A.primes[n0 + (it - e0)];
+      const uint32_t p1 = e1i & 0xFFFFFFu, r1 = e1i >> 24;
+      const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab);
+      if (offl < hi) {
+        uint32_t j = (uint32_t)offl;
+        if (j < lo) {
+          const uint32_t back = mod_u32(lo - j, p1);
+          j = lo + (back ? p1 - back : 0u);
+        }
+        if (j < hi) cross_class(base, j, p1, hi, tab[64 + r1 * 8 + c]);
+      }
+    } else {
+      // tier 3: one prime per lane, at most four crossings per class
+      if (e) cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
     }
-  }
-  // Tiers 2-4 load the next turn's prime before working on this one, so the
-  // table read's latency overlaps the crossings.
-  uint32_t e_next = (n1 + 64 * wave + lane < n2) ? A.primes[n1 + 64 * wave + lane] : 0u;
-  for (uint32_t b = n1 + 64 * wave; b < n2; b += 64 * NW) {
-    const uint32_t i = b + lane;
-    const uint32_t e = e_next;
-    e_next = (i + 64 * NW < n2) ? A.primes[i + 64 * NW] : 0u;
-    if (i >= n2) continue;
-    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-    cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
-  }
-  e_next = (n2 + 64 * wave + lane < n3) ? A.primes[n2 + 64 * wave + lane] : 0u;
-  for (uint32_t b = n2 + 64 * wave; b < n3; b += 64 * NW) {
-    const uint32_t i = b + lane;
-    const uint32_t e = e_next;
-    e_next = (i + 64 * NW < n3) ? A.primes[i + 64 * NW] : 0u;
-    if (i >= n3) continue;
-    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
-    cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
+    it = nit;
+    e = ne;
   }
   // tier 4 (p >= seg_bytes): at most one crossing per class.  The classes'
   // first multiples cc_c (byte offsets within one period p) increase with c,
   // so the classes that hit the window [rk, rk + nbytes) mod p form one run
   // (two when the window wraps past p): a three-step search finds its start
-  // and only the run is visited.
-  e_next = (n3 + 64 * wave + lane < np) ? A.primes[n3 + 64 * wave + lane] : 0u;
+  // and only the run is visited.  The next turn's prime is loaded before
+  // working on this one, so the table read's latency overlaps the crossings.
+  uint32_t e_next = (n3 + 64 * wave + lane < np) ? A.primes[n3 + 64 * wave + lane] : 0u;
   for (uint32_t b = n3 + 64 * wave; b < np; b += 64 * NW) {
     const uint32_t i = b + lane;
-    const uint32_t e = e_next;
+    const uint32_t e4 = e_next;
     e_next = (i + 64 * NW < np) ? A.primes[i + 64 * NW] : 0u;
     if (i >= np) continue;
-    const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
+    const uint32_t p = e4 & 0xFFFFFFu, r = e4 >> 24;
     const uint32_t rk = mod_u64(k0, p);
     if (p >= plow) {  // the p*p rule can bite: general path
       cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
