@@ -46,7 +46,18 @@ LDS_PEAK_OPS = 256 * 32 * 2.4e9        # gfx950: 256 CUs, 128 B/clk/CU LDS = 32 
 PI_1E11 = 4_118_054_813                # A006880
 
 
-def crossings(N: int, primes_upto) -> int:
+def small_primes_upto(x: int):
+    """Primes <= x by a plain numpy sieve of Eratosthenes (x is sqrt(N): small)."""
+    import numpy as np
+    flags = np.ones(x + 1, dtype=bool)
+    flags[:2] = False
+    for q in range(2, math.isqrt(x) + 1):
+        if flags[q]:
+            flags[q * q::q] = False
+    return [int(v) for v in np.nonzero(flags)[0]]
+
+
+def crossings(N: int, primes_upto=small_primes_upto) -> int:
     """C(N) = sum over primes 7 <= p <= sqrt(N) of #{m in [p, N/p] : gcd(m, 30) = 1}."""
     w = (1, 7, 11, 13, 17, 19, 23, 29)
 
@@ -229,13 +240,7 @@ def main():
     if N == 10**11:
         exact = exact and r["prime_count"] == PI_1E11
 
-    from oracle import oracle as O
-    O.build()
-
-    def primes_upto(x):
-        return O.primes_in_range(2, x)
-
-    C = crossings(N, primes_upto)
+    C = crossings(N)
     sieve_ms = sum(k["sieve_ms"] for k in kernel_ms) / len(kernel_ms)
     algo_ops = C + r["prime_count"]
     achieved = algo_ops / (sieve_ms / 1e3)
