@@ -88,7 +88,7 @@ def cpu_baseline(sample_n: int, reps: int):
                       f"median of {reps}: {dt:.2f} s wall, pi = {s['prime_count']}"}
 
 
-FIXTURE_C2 = ROOT / "tests" / "golden" / "ranges_c2.json"
+FIXTURE_RANGES_4096 = ROOT / "tests" / "golden" / "ranges_4096.json"
 
 
 def fnv_digests(hist):
@@ -106,7 +106,7 @@ def fnv_digests(hist):
 
 def bench_batch(st, reps: int):
     import numpy as np
-    gold = json.loads(FIXTURE_C2.read_text())["queries"]
+    gold = json.loads(FIXTURE_RANGES_4096.read_text())["queries"]
     a = [g[0] for g in gold]
     b = [g[1] for g in gold]
     st.batch_ranges(a, b, as_numpy=True)  # warm: plan capture
@@ -158,7 +158,7 @@ def bench_serve(n: int):
     return out
 
 
-FIXTURE_C4 = ROOT / "tests" / "golden" / "sweep_1e12.json"
+FIXTURE_SWEEP_1E12 = ROOT / "tests" / "golden" / "sweep_1e12.json"
 
 
 def bench_large(st):
@@ -166,7 +166,7 @@ def bench_large(st):
     mod-4/mod-3 tallies every 1e10), Goldbach existence for every even n in
     [4, 1e9] and partition counts at n = j * 1e6; exact against the fixture."""
     import voss_amd
-    f = json.loads(FIXTURE_C4.read_text())
+    f = json.loads(FIXTURE_SWEEP_1E12.read_text())
     N = f["N"]
     st.full_stats(N, cp_interval=f["checkpoint_interval"], as_numpy=True)  # warm
     t0 = time.perf_counter()

@@ -1,11 +1,11 @@
-"""Writes ranges_c2.json: the oracle's answers for the batched-range workload.
+"""Writes ranges_4096.json: the oracle's answers for the batched-range workload.
 
 Workload (seed 0): 4096 queries, a = floor(10**U(6, 13)), w = floor(10**U(3, 9)),
 b = a + w - 1, drawn with numpy.random.default_rng(0) (a's first, then w's).
 Per query the fixture keeps count, first, last and a 64-bit FNV-1a digest of
 the 257-bin gap histogram (bins [i] = gaps of 2i for i = 1..256, [0] = other).
 
-    python tests/golden/make_ranges_c2.py   (several minutes on 8 cores)
+    python tests/golden/make_ranges_4096.py   (several minutes on 8 cores)
 """
 
 import json
@@ -44,7 +44,7 @@ def main():
         out.append([a, b, r["count"], r["first"], r["last"], hist_digest(r["hist"])])
         if i % 256 == 0:
             print(i, flush=True)
-    Path(__file__).with_name("ranges_c2.json").write_text(json.dumps(
+    Path(__file__).with_name("ranges_4096.json").write_text(json.dumps(
         {"workload": "4096 ranges, seed 0, a=10^U(6,13), w=10^U(3,9)", "fields": ["a", "b", "count", "first", "last", "hist_fnv1a64"],
          "queries": out}, separators=(",", ":")))
 

@@ -6,9 +6,9 @@ import random
 import sys
 from pathlib import Path
 
-GOLD = Path(__file__).parent / "golden" / "ranges_c2.json"
+GOLD = Path(__file__).parent / "golden" / "ranges_4096.json"
 sys.path.insert(0, str(GOLD.parent))
-from make_ranges_c2 import hist_digest, queries  # noqa: E402
+from make_ranges_4096 import hist_digest, queries  # noqa: E402
 
 
 def test_fixture_queries_and_sample(oracle):

@@ -1,6 +1,6 @@
 """Batched ranges on the GPU at full size (4096 queries, seed 0) against the
-oracle's committed answers (tests/golden/ranges_c2.json, written by
-tests/golden/make_ranges_c2.py), and batch == plain per-query path."""
+oracle's committed answers (tests/golden/ranges_4096.json, written by
+tests/golden/make_ranges_4096.py), and batch == plain per-query path."""
 
 import json
 import sys
@@ -9,12 +9,12 @@ from pathlib import Path
 import pytest
 
 pytestmark = pytest.mark.gpu
-GOLD = Path(__file__).parent / "golden" / "ranges_c2.json"
+GOLD = Path(__file__).parent / "golden" / "ranges_4096.json"
 sys.path.insert(0, str(GOLD.parent))
-from make_ranges_c2 import hist_digest, queries  # noqa: E402
+from make_ranges_4096 import hist_digest, queries  # noqa: E402
 
 
-def test_c2_batch_matches_golden(voss):
+def test_4096_ranges_batch_matches_golden(voss):
     gold = json.loads(GOLD.read_text())["queries"]
     qs = queries()
     assert [(g[0], g[1]) for g in gold] == qs
@@ -25,7 +25,7 @@ def test_c2_batch_matches_golden(voss):
             assert hist_digest(r["hist"]) == g[5], (g[0], g[1], rep)
 
 
-def test_c2_plain_equals_batch(voss):
+def test_4096_ranges_plain_equals_batch(voss):
     qs = queries()[:96]
     batch = voss.stats.batch_ranges([a for a, _ in qs], [b for _, b in qs])
     for q, r in zip(qs, batch):
