@@ -16,7 +16,7 @@ constexpr int kPresieveGroups = 1;
 constexpr uint32_t kPresievePeriods[kPresieveGroups] = {7u * 11u * 13u * 17u * 19u};
 constexpr uint32_t kFirstSievingPrime = 23;
 // Primes below this are sieved by tier 0 (a half-wave per prime and class).
-constexpr uint32_t kTier0P = 256;
+constexpr uint32_t kTier0P = 192;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
 // LDS gap histogram: bins index gap/2 and cover every gap a batched query
 // bins (2..512); longer gaps fall into the last bin and take an exact slow

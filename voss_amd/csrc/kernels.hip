@@ -346,7 +346,7 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
 }
 
 // Cross off every sieving prime >= 23 with p*p <= the segment's top value.
-//   tier 0 (p < 256): a half-wave per (prime, class), its 32 lanes splitting
+//   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes splitting
 //     the class's crossings into runs whose LDS banks never collide;
 //   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
 //     lanes are 8 classes x 8 sub-ranges of the segment;
