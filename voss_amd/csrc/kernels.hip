@@ -678,7 +678,7 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
-  const uint32_t copy = lane >> 3;
+  const uint32_t copy = lane & 7;
   const uint32_t wave_bytes = A.seg_bytes / NW;
   const uint32_t M41 = (g.k0 & 1) ? 0x99669966u : 0x66996699u;
   const uint32_t M31 = 0x2B2B2B2Bu;
