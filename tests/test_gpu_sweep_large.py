@@ -63,6 +63,17 @@ def test_goldbach_1e9(voss):
     assert voss.stats.goldbach_counts(ns) == f["goldbach_counts"]
 
 
+# P(6)/P(2), the ratio of consecutive-prime gaps 6 and 2, as the frozen v7
+# program records its trend (src/voss_w30_v7.cu:524-534, 4 decimals)
+V7_P6_P2 = {10**9: 1.7783, 10**10: 1.8018, 10**11: 1.8208, 10**12: 1.8366, 10**13: 1.8497}
+
+
+@pytest.mark.parametrize("N", [10**9, 10**10, 10**11, 10**12])
+def test_v7_gap_repulsion_trend(voss, N):
+    h = voss.stats.full_stats(N)["hist"]
+    assert abs(h[6] / h[2] - V7_P6_P2[N]) < 5e-5
+
+
 def test_prime_count_1e13(voss):
     # library/STATE.md:102-103: pi(1e13) = 346,065,536,839
     assert voss.primes.prime_count(10**13) == 346_065_536_839
@@ -76,6 +87,7 @@ def test_full_stats_1e13_known(voss):
     assert g["hist"][2] == 15_834_664_872
     assert g["records"][-1] == (7_177_162_611_713 + 674, 674)
     assert g["max_gap"] == 674
+    assert abs(g["hist"][6] / g["hist"][2] - V7_P6_P2[10**13]) < 5e-5
     h = g["hist"]
     assert sum(h) == g["prime_count"] - 1 and sum(i * c for i, c in enumerate(h)) == g["last_prime"] - 2
     assert g["pi_4_1"] + g["pi_4_3"] + 1 == g["prime_count"] == g["pi_3_1"] + g["pi_3_2"] + 1

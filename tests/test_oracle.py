@@ -87,6 +87,9 @@ def test_known_gap_counts(oracle):
     assert s["large_gaps"] == []
     mean = sum(g * h for g, h in enumerate(s["hist"])) / sum(s["hist"])
     assert round(mean, 4) == 19.6666
+    # gap repulsion P(6)/P(2) at 1e9 as the frozen v7 program records it
+    # (src/voss_w30_v7.cu:524-534, 4 decimals)
+    assert abs(s["hist"][6] / s["hist"][2] - 1.7783) < 5e-5
 
 
 def test_known_chebyshev_1e4(oracle):
