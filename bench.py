@@ -8,10 +8,11 @@ benchmark, README.md:79, extended with the checkpoint tallies and records).
 The input is just N: nothing crosses PCIe before the timed region.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the LDS
-segment sieve): algorithmic LDS dword operations per launch (crossings C(N)
-plus one histogram update per prime, C(N) from the closed form) divided by
-the kernel's HIP-event time on the engine stream; peak = 256 CUs x 32 LDS
-dword ops/clk x 2.4 GHz.  `cpu_baseline` times the oracle (a multithreaded
+segment sieve): algorithmic LDS bytes per launch -- one 4-byte dword per
+crossing (C(N), from the closed form) and per gap-histogram update (one per
+prime) -- divided by the kernel's HIP-event time on the engine stream;
+peak = 256 CUs x 128 B/clk x 2.4 GHz of LDS bandwidth.  The bound is the LDS,
+not HBM or MFMA: the bitmap never leaves it.  `cpu_baseline` times the oracle (a multithreaded
 C port of the same algorithm) on a bounded sample of the same workload.
 
 Two more sections ride on the same line:
@@ -42,7 +43,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 A100_INTS_PER_S_AT_1E11 = 1e11 / 5.36  # reference README.md:79 (A100), the published headline
-LDS_PEAK_OPS = 256 * 32 * 2.4e9        # gfx950: 256 CUs, 128 B/clk/CU LDS = 32 dword ops/clk, 2.4 GHz
+LDS_PEAK_BYTES = 256 * 128 * 2.4e9     # gfx950: 256 CUs x 128 B/clk/CU of LDS x 2.4 GHz
 PI_1E11 = 4_118_054_813                # A006880
 
 
@@ -243,7 +244,7 @@ def main():
     C = crossings(N)
     sieve_ms = sum(k["sieve_ms"] for k in kernel_ms) / len(kernel_ms)
     algo_ops = C + r["prime_count"]
-    achieved = algo_ops / (sieve_ms / 1e3)
+    achieved = 4 * algo_ops / (sieve_ms / 1e3)  # bytes/s: one LDS dword per algorithmic op
     traffic = None
     tf = ROOT / "profiles" / "traffic_full_stats_1e11.json"
     if tf.exists():
@@ -272,10 +273,11 @@ def main():
                       "stitch_and_rescan": sum(k["stitch_ms"] for k in kernel_ms) / len(kernel_ms),
                       "device_total": sum(k["total_ms"] for k in kernel_ms) / len(kernel_ms),
                       "segments": kernel_ms[-1]["segments"]},
-        "roofline": {"bound": "lds", "achieved": achieved / 1e9, "peak": LDS_PEAK_OPS / 1e9,
-                     "unit": "Gop/s (LDS dword ops)", "frac": achieved / LDS_PEAK_OPS,
+        "roofline": {"bound": "lds", "achieved": achieved / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
+                     "unit": "GB/s", "frac": achieved / LDS_PEAK_BYTES,
                      "traffic": traffic,
-                     "algorithmic_ops_per_launch": algo_ops, "crossings": C,
+                     "algorithmic_ops_per_launch": algo_ops, "algorithmic_lds_bytes_per_launch": 4 * algo_ops,
+                     "crossings": C,
                      "note": "LDS-resident sieve: HBM carries only tables and per-segment summaries"},
     }
     if not args.no_batch:
