@@ -103,3 +103,15 @@ def test_serving_trace_is_deterministic(voss):
     nnp = r1["a"][np.isin(r1["kind"], [serve.NTH, serve.NEXT, serve.PREV])]
     assert nnp.min() >= 1 and nnp.max() <= 10**12
     assert (r1["n"][r1["kind"] == serve.IS_PRIME] == 64).all()
+
+
+def test_c_program_argument_checks():
+    """The plain-C ABI test (tests/c/test_capi.c) in its no-device mode: error
+    codes, untouched out-parameters, last-error text, NULL releases."""
+    import subprocess
+    exe = ROOT / "tests" / "c" / "test_capi"
+    if not exe.exists():
+        subprocess.run(["make", "-C", str(ROOT / "tests" / "c")], check=True, capture_output=True)
+    r = subprocess.run([str(exe), "args"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL TESTS PASSED" in r.stdout
