@@ -35,6 +35,42 @@ def test_full_stats_vs_oracle(voss, oracle, N):
     assert g["checkpoints"] == o["checkpoints"]
 
 
+def test_full_stats_random_vs_oracle(voss, oracle):
+    # seeded N anywhere in [1e5, 3e9] with random checkpoint intervals and
+    # large-gap thresholds: every output against the oracle
+    rng = random.Random(2024)
+    for _ in range(10):
+        N = rng.randrange(10**5, 3 * 10**9)
+        cp = rng.choice([0, rng.randrange(10**7, max(N, 10**7 + 1))])
+        thr = rng.choice([500, rng.randrange(250, 500)])
+        g = voss.stats.full_stats(N, cp_interval=cp, large_threshold=thr)
+        o = oracle.full_stats(N, cp_interval=cp, large_threshold=thr)
+        for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"):
+            assert g[k] == o[k], (N, cp, thr, k)
+        assert g["hist"] == o["hist"], (N, cp, thr)
+        assert g["records"] == o["records"], (N, cp, thr)
+        assert g["large_gaps"] == o["large_gaps"], (N, cp, thr)
+        assert g["checkpoints"] == o["checkpoints"], (N, cp, thr)
+
+
+def test_ranges_near_the_top_vs_oracle(voss, oracle):
+    # queries between 1e13 and the API's 1e14 ceiling: sieving primes up to
+    # 1e7, most of them in the sparse tier of a short segment
+    rng = random.Random(99)
+    qs = []
+    for _ in range(24):
+        a = rng.randrange(10**13, 10**14 - 10**7)
+        qs.append((a, a + int(10 ** rng.uniform(1, 6.5)) - 1))
+    qs += [(10**14 - 10**6, 10**14), (10**14 - 100, 10**14), (10**14 - 30, 10**14 - 29)]
+    got = voss.stats.batch_ranges([q[0] for q in qs], [q[1] for q in qs])
+    for (a, b), g in zip(qs, got):
+        o = oracle.range_stats(a, b)
+        assert (g["count"], g["first"], g["last"]) == (o["count"], o["first"], o["last"]), (a, b)
+        assert g["hist"] == o["hist"], (a, b)
+        assert voss.stats.range_stats(a, b) == g, (a, b)
+    assert voss.primes.in_range(10**14 - 10**5, 10**14) == oracle.primes_in_range(10**14 - 10**5, 10**14)
+
+
 def test_full_stats_large_gap_threshold(voss, oracle):
     # a low threshold makes the large-gap list long, exercising the append path
     g = voss.stats.full_stats(10**8, large_threshold=150)
