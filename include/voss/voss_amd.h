@@ -1,6 +1,6 @@
 /* voss_amd.h — entry points beyond the reference C ABI.
  *
- * These have no counterpart in the reference (library/include/voss/*.h);
+ * These have no counterpart in the reference (library/include/voss/voss.h and voss_primes.h);
  * they expose what the MI355X engine computes in one pass (checkpointed
  * Chebyshev tallies, maximal-gap records, the full histogram), batched and
  * served range queries, GPU Miller-Rabin and Goldbach counts, and the engine

@@ -43,10 +43,7 @@ __device__ __forceinline__ uint32_t w30_const(int o) {
   return o == 0 ? 1 : o == 1 ? 7 : o == 2 ? 11 : o == 3 ? 13 : o == 4 ? 17 : o == 5 ? 19 : o == 6 ? 23 : 29;
 }
 
-// LDS atomic AND without return (ds_and_b32).
-__device__ __forceinline__ void lds_and(uint32_t* p, uint32_t m) {
-  __hip_atomic_fetch_and(p, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
+// LDS atomic add without return (ds_add_u32).
 __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -54,10 +51,6 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
 __device__ __forceinline__ void lds_add(lds_u32* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-// LDS atomic add at a byte address (ds_add_u32).
-__device__ __forceinline__ void lds_add_at(uint32_t addr, uint32_t v) {
-  __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // k mod p for k < 2^52, p < 2^24, from a Newton-refined FP64 reciprocal.
@@ -495,8 +488,8 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
     // counts, every class with W > w does not, and only W == w needs a look.
     uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
-    if (__umul24(w, q30) > rk) --w;
-    if (__umul24(w + 1, q30) <= rk) ++w;
+    if ((uint32_t)__umul24(w, q30) > rk) --w;
+    if ((uint32_t)__umul24(w + 1, q30) <= rk) ++w;
     const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
     uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
     if (c < 8 && cc(c) < rk) ++c;
