@@ -33,6 +33,31 @@ def test_headers_declare_the_reference_surface():
         assert must in names
 
 
+# The reference library's 39 exported entry points (library/include/voss/
+# voss.h:19-25, voss_primes.h:13-181, voss_numbers.h:16-45), the set its CI
+# checks with `nm -D`: every one must be declared here and exported.
+REFERENCE_EXPORTS = (
+    ["voss_strerror", "voss_get_last_error", "voss_free"]
+    + ["voss_primes_" + n for n in (
+        "prime_count", "in_range", "in_range_with_limit", "nth", "next", "prev", "is_prime",
+        "factorize", "is_mersenne_prime", "is_fermat_prime",
+        "goldbach_count", "goldbach_partitions", "goldbach_test_upto")]
+    + ["voss_primes_ctx_" + n for n in (
+        "new", "free", "prime_count", "twins", "cousin", "sexy", "statistics", "chebyshev_bias",
+        "large_gaps_count", "large_gaps_get", "export_csv", "set_progress", "sophie_germain",
+        "sophie_germain_upto")]
+    + ["voss_numbers_" + n for n in ("phi", "tau", "sigma", "mu", "gcd", "lcm", "fibonacci", "factorial")]
+)
+
+
+def test_reference_exports_declared_and_exported():
+    assert len(REFERENCE_EXPORTS) == 39 == len(set(REFERENCE_EXPORTS))
+    names = declared()
+    assert [n for n in REFERENCE_EXPORTS if n not in names] == []
+    lib = ctypes.CDLL(str(ROOT / "voss_amd" / "libvoss.so"))
+    assert [n for n in REFERENCE_EXPORTS if not hasattr(lib, n)] == []
+
+
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(ROOT / "voss_amd" / "libvoss.so"))
     missing = [n for n in sorted(declared()) if not hasattr(lib, n)]
