@@ -196,3 +196,27 @@ def test_ticket_of_an_earlier_server(voss, serve):
         with pytest.raises(voss.VossError):
             b.wait(t2)
         assert int(b.run_one(serve.request(serve.PREV, 100))["value"]) == 97
+
+
+def test_cached_ranges(voss, oracle, serve):
+    # ranges inside the server's resident bitmap of [7, goldbach_max] are read
+    # from it instead of sieved: every edge against the oracle and against the
+    # sieving path (stats.range_stats), including b at the cache limit and one
+    # past it (sieved again)
+    top = 2 * 10**6
+    qs = [(2, 10), (2, 1000), (5, 7), (7, 7), (0, 0), (1, 1), (24, 28), (999_990, 1_000_010),
+          (top - 100, top), (top - 100, top + 1), (top, top + 50), (30 * 4096 - 5, 30 * 4096 + 5),
+          (7, top), (2, top)]
+    rng = np.random.default_rng(40)
+    for _ in range(40):
+        a = int(rng.integers(0, top))
+        qs.append((a, min(top, a + int(10 ** rng.uniform(0, 6.3)))))
+    with serve.Server(nth_max=10**4, goldbach_max=top) as srv:
+        for a, b in qs:
+            r = srv.run_one(serve.request(serve.RANGE, a, b))
+            o = oracle.range_stats(max(a, 2), b) if b >= 2 else {"count": 0, "first": 0, "last": 0, "hist": [0] * 257}
+            got = (int(r["status"]), int(r["value"]), int(r["first"]), int(r["last"]))
+            assert got == (0, o["count"], o["first"], o["last"]), (a, b)
+            assert list(r["hist"]) == list(o["hist"]), (a, b)
+            p = voss.stats.range_stats(a, b)
+            assert (p["count"], p["first"], p["last"], p["hist"]) == (got[1], got[2], got[3], list(r["hist"])), (a, b)

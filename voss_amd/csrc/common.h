@@ -56,6 +56,8 @@ struct QueryDesc {
   uint64_t k_lo, k_hi;    // byte range [k_lo, k_hi)
   uint32_t seg_first;     // index of the query's first segment in the work-list
   uint32_t nseg;
+  uint32_t from_cache;    // 1: the segments' bytes come from the resident cache bitmap (no crossing-off)
+  uint32_t pad;
 };
 
 enum KernelMode : int {
@@ -105,6 +107,9 @@ struct SieveArgs {
   unsigned long long* qcount;   // per query
   // bitmap mode
   uint8_t* bitmap;              // byte (k - k_base) of the sieved range
+  // batch mode: resident sieved bitmap of [7, cache_hi] from byte 0, read by
+  // queries flagged from_cache instead of sieving their segments
+  const uint8_t* cache;
 };
 
 }  // namespace voss

@@ -80,6 +80,8 @@ struct BatchPlan {
   uint32_t seg_bytes = 0, tier1_p = 0;
   int block = 0, wgs_per_cu = 0;
   uint64_t table_limit = 0;
+  const uint8_t* cache = nullptr;  // resident sieved bitmap of [7, cache_hi] (null: none)
+  uint64_t cache_hi = 0;
   size_t in_bytes = 0, out_bytes = 0;
   void* h_in = nullptr;   // pinned: header | queries | segment->query map
   void* h_out = nullptr;  // pinned: count | first | last | histograms
@@ -151,7 +153,10 @@ class Engine {
   };
   BatchLayout layout_batch(const uint64_t* a, const uint64_t* b, size_t n, uint32_t seg_bytes) const;
   // capture a batch pipeline for up to qcap queries / scap segments on stream st
-  std::unique_ptr<BatchPlan> make_batch_plan(uint32_t qcap, uint32_t scap, const TuneConfig& c, hipStream_t st);
+  // queries with b <= cache_hi read their bytes from cache (a sieved bitmap of
+  // [7, cache_hi] from byte 0, as bitmap_range writes it) instead of sieving
+  std::unique_ptr<BatchPlan> make_batch_plan(uint32_t qcap, uint32_t scap, const TuneConfig& c, hipStream_t st,
+                                             const uint8_t* cache = nullptr, uint64_t cache_hi = 0);
   static void stage_batch(BatchPlan& P, const BatchLayout& L, const uint64_t* a, const uint64_t* b);
   // results (including the seeds 2, 3, 5) into out[0..n); P may be null when L.nq == 0
   static void read_batch(const BatchPlan* P, const BatchLayout& L, const uint64_t* a, const uint64_t* b, size_t n,

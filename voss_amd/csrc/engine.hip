@@ -591,7 +591,7 @@ static uint32_t pow2_at_least(uint64_t v, uint32_t lo) {
 }
 
 std::unique_ptr<BatchPlan> Engine::make_batch_plan(uint32_t qcap, uint32_t scap, const TuneConfig& c,
-                                                   hipStream_t st) {
+                                                   hipStream_t st, const uint8_t* cache, uint64_t cache_hi) {
   std::unique_ptr<BatchPlan> up(new BatchPlan());
   BatchPlan& P = *up;
   P.qcap = qcap;
@@ -601,6 +601,8 @@ std::unique_ptr<BatchPlan> Engine::make_batch_plan(uint32_t qcap, uint32_t scap,
   P.wgs_per_cu = c.wgs_per_cu;
   P.tier1_p = c.tier1_p;
   P.table_limit = table_limit_;
+  P.cache = cache;
+  P.cache_hi = cache ? cache_hi : 0;
   P.in_bytes = 64 + (size_t)qcap * sizeof(QueryDesc) + (size_t)scap * 4;
   P.out_bytes = (size_t)qcap * 24 + (size_t)qcap * kQueryBins * 4;
   hip_check(hipHostMalloc(&P.h_in, P.in_bytes, hipHostMallocDefault), "hipHostMalloc");
@@ -624,6 +626,7 @@ std::unique_ptr<BatchPlan> Engine::make_batch_plan(uint32_t qcap, uint32_t scap,
   A.seg_query = d_sq;
   A.qhist = d_qh;
   A.qcount = d_qc;
+  A.cache = cache;
   BatchStitchArgs B;
   B.queries = d_q;
   B.summaries = P.d_sum.as<SegSummary>();
@@ -696,6 +699,8 @@ void Engine::stage_batch(BatchPlan& P, const BatchLayout& L, const uint64_t* a, 
     Q.k_hi = b[i] / 30 + 1;
     Q.seg_first = seg;
     Q.nseg = (uint32_t)((Q.k_hi - Q.k_lo + S - 1) / S);
+    Q.from_cache = (P.cache && Q.b <= P.cache_hi) ? 1u : 0u;
+    Q.pad = 0;
     for (uint32_t k = 0; k < Q.nseg; ++k) hsq[seg + k] = q;
     seg += Q.nseg;
   }

@@ -130,6 +130,7 @@ struct SegGeom {
   uint32_t vlo, vhi;      // valid bit range [vlo, vhi)
   uint64_t lo_val, hi_val;
   uint32_t query;
+  uint32_t from_cache;    // batch mode: bytes come from A.cache, no crossing-off
 };
 
 // Gaps between consecutive primes inside one bitmap byte x, in order, as
@@ -202,6 +203,42 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
         w[i] &= keep >= 4 ? 0xFFFFFFFFu : ((1u << (8 * keep)) - 1u);
       }
       v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
+    }
+    *reinterpret_cast<uint4*>(seg8 + b0) = v;
+  }
+}
+
+// Load the segment from the resident cache bitmap (sieved bytes from k = 0)
+// instead of the presieve pattern: the bytes are final, no crossing-off
+// follows.  As init_segment, it zero-fills up to the next 1 KiB and clears
+// bytes past nbytes; a chunk reads at most 20 bytes from its start, and only
+// chunks that begin inside the segment read at all.
+template <int BS>
+__device__ void cache_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
+  uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
+  const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
+  const uint32_t sh = (uint32_t)(g.k0 & 3);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(A.cache) + (g.k0 >> 2);
+  for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
+    const uint32_t b0 = 16 * t;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (b0 < g.nbytes) {
+      const uint32_t* p = src + 4 * t;
+      const uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
+      v.x = __builtin_amdgcn_alignbyte(a1, a0, sh);
+      v.y = __builtin_amdgcn_alignbyte(a2, a1, sh);
+      v.z = __builtin_amdgcn_alignbyte(a3, a2, sh);
+      v.w = __builtin_amdgcn_alignbyte(a4, a3, sh);
+      if (b0 + 16 > g.nbytes) {
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t wb = b0 + 4 * i;
+          const uint32_t keep = (wb >= g.nbytes) ? 0u : (g.nbytes - wb >= 4 ? 4u : g.nbytes - wb);
+          w[i] &= keep >= 4 ? 0xFFFFFFFFu : ((1u << (8 * keep)) - 1u);
+        }
+        v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
+      }
     }
     *reinterpret_cast<uint4*>(seg8 + b0) = v;
   }
@@ -810,8 +847,10 @@ __device__ __forceinline__ SegGeom geometry(const SieveArgs& A, uint32_t s, int 
     g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
     g.lo_val = Q.a;
     g.hi_val = Q.b;
+    g.from_cache = Q.from_cache;
   } else {
     g.query = 0;
+    g.from_cache = 0;
     g.k0 = A.k_base + (uint64_t)(A.seg_base + s) * A.seg_bytes;
     uint64_t rem = A.k_end - g.k0;
     g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
@@ -856,9 +895,14 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       if (cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
       cur_query = g.query;
     }
-    init_segment<BS>(A, lds, g);
+    const bool cached = MODE == kModeBatch && g.from_cache;  // uniform per workgroup
+    if (cached) {
+      cache_segment<BS>(A, lds, g);
+    } else {
+      init_segment<BS>(A, lds, g);
+    }
     __syncthreads();
-    sieve_segment<BS>(A, lds, g);
+    if (!cached) sieve_segment<BS>(A, lds, g);
     __syncthreads();
     if (MODE == kModeBitmap) {
       const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
@@ -1126,6 +1170,7 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   const uint32_t s = item.seg;
   SegGeom g;
   g.query = 0;
+  g.from_cache = 0;
   g.k0 = A.k_base + (uint64_t)s * A.seg_bytes;
   const uint64_t rem = A.k_end - g.k0;
   g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
@@ -1238,6 +1283,7 @@ __global__ void __launch_bounds__(BS) select_kernel(SieveArgs A, const uint32_t*
   const uint32_t rank = ranks[blockIdx.x];
   SegGeom g;
   g.query = 0;
+  g.from_cache = 0;
   g.k0 = (uint64_t)s * A.seg_bytes;
   const uint64_t rem = A.k_end - g.k0;
   g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;

@@ -215,7 +215,10 @@ void Server::launch(Slot& s) {
         if (p->qcap == qcap && p->scap == scap) s.plan = p.get();
       if (!s.plan) {
         if (s.plans.size() >= 6) s.plans.erase(s.plans.begin());
-        s.plans.push_back(eng_.make_batch_plan(qcap, scap, rcfg_, s.st));
+        // ranges inside the resident Goldbach bitmap read it instead of sieving
+        const bool cache = cfg_.goldbach_max >= 4 && gb_bitmap_.p;
+        s.plans.push_back(eng_.make_batch_plan(qcap, scap, rcfg_, s.st, cache ? gb_bitmap_.as<uint8_t>() : nullptr,
+                                               cache ? cfg_.goldbach_max : 0));
         s.plan = s.plans.back().get();
       }
       Engine::stage_batch(*s.plan, s.layout, s.ra.data(), s.rb.data());
