@@ -119,7 +119,7 @@ namespace {
 
 // misc word layout
 constexpr uint32_t kMiscItem = 0;       // current work item
-constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work item of the segment
+constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work items: tiers 0-3 (word 1), tier 4 (word 2)
 constexpr uint32_t kMiscWave = 8;       // 8 words per wave below
 // per wave: [0] first, [1] last, [2] count, [3] c41, [4] c31, [5] maxgap,
 //           [6] lower count, [7] lower c41 ... (lower c31 in [kMiscWave + 8*NW])
@@ -174,7 +174,7 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
 #pragma unroll
   for (int k = 0; k < kPresieveGroups; ++k) off[k] = (uint32_t)(g.k0 % kPresievePeriods[k]);
   const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
-  if (threadIdx.x == 0) lds[kLdsMisc + kMiscSieveCtr] = 0;  // sieve_segment's work counter
+  if (threadIdx.x < 2) lds[kLdsMisc + kMiscSieveCtr + threadIdx.x] = 0;  // sieve_segment's work counters
   for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
     uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
@@ -426,8 +426,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   // (prime, class pair) items, tier-1 primes, then the 64-prime chunks of
   // tier 3.  Each wave takes its next item from an LDS counter (cleared by
   // init_segment), so waves that drew heavy items take fewer and all of them
-  // reach tier 4 together.  Tier 4's chunks cost about the same each (and a
-  // short segment has thousands of them) and are dealt round-robin.
+  // reach tier 4 together.  Tier 4 has its own counter below.
   const uint32_t e2 = (n2 - n1 + 63) >> 6;
   const uint32_t e0 = e2 + 4 * n0;
   const uint32_t e1 = e0 + (n1 - n0);
@@ -502,48 +501,76 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   // first multiples cc_c (byte offsets within one period p) increase with c,
   // so the classes that hit the window [rk, rk + nbytes) mod p form one run
   // (two when the window wraps past p): a three-step search finds its start
-  // and only the run is visited.  The next turn's prime is loaded before
-  // working on this one, so the table read's latency overlaps the crossings.
-  uint32_t e_next = (n3 + 64 * wave + lane < np) ? A.primes[n3 + 64 * wave + lane] : 0u;
-  for (uint32_t b = n3 + 64 * wave; b < np; b += 64 * NW) {
-    const uint32_t i = b + lane;
-    const uint32_t e4 = e_next;
-    e_next = (i + 64 * NW < np) ? A.primes[i + 64 * NW] : 0u;
-    if (i >= np) continue;
-    const uint32_t p = e4 & 0xFFFFFFu, r = e4 >> 24;
-    const uint32_t rk = mod_u64(k0, p);
-    if (p >= plow) {  // the p*p rule can bite: general path
-      cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
-      continue;
-    }
-    const uint32_t q30 = p / 30;
-    const uint2 fr = *reinterpret_cast<const uint2*>(tab + r * 8);
-    const uint2 br = *reinterpret_cast<const uint2*>(tab + 64 + r * 8);
-    auto cc = [&](uint32_t c) { return __umul24(q30, w30(c)) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
-    auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(br.y, br.x, c | 0x0C0C0C00u) & 7u; };
-    // c = number of classes with cc_c < rk.  cc_c = q30 W[c] + frac_c with
-    // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
-    // counts, every class with W > w does not, and only W == w needs a look.
-    uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
-    if ((uint32_t)__umul24(w, q30) > rk) --w;
-    if ((uint32_t)__umul24(w + 1, q30) <= rk) ++w;
-    const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
-    uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
-    if (c < 8 && cc(c) < rk) ++c;
-    const uint32_t end = rk + nbytes;
-    for (; c < 8; ++c) {
-      const uint32_t x = cc(c);
-      if (x >= end) break;
-      cross_one(base, x - rk, bit_of(c));
-    }
-    if (end > p) {
-      const uint32_t end2 = end - p;
-      for (uint32_t c2 = 0; c2 < 8; ++c2) {
-        const uint32_t x = cc(c2);
-        if (x >= end2) break;
-        cross_one(base, x + p - rk, bit_of(c2));
+  // and only the run is visited.  Items are groups of g4 64-prime chunks
+  // (about eight per wave: fine enough to balance the waves, few enough to
+  // bound the counter traffic of a short segment with thousands of chunks)
+  // taken from a second LDS counter; the primes of the next chunk, and of
+  // the next item's first chunk, are loaded ahead.
+  const uint32_t c4 = (np - n3 + 63) >> 6;
+  const uint32_t g4 = max(1u, (c4 + 8 * NW - 1) / (8 * NW));
+  const uint32_t items4 = (c4 + g4 - 1) / g4;
+  auto take4 = [&]() -> uint32_t {
+    uint32_t v = 0;
+    if (lane == 0)
+      v = __hip_atomic_fetch_add((lds_u32*)(uintptr_t)((kLdsMisc + kMiscSieveCtr + 1) * 4), 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __builtin_amdgcn_readlane(v, 0);
+  };
+  auto first4 = [&](uint32_t t) -> uint32_t {
+    const uint32_t i = n3 + 64 * g4 * t + lane;
+    return (t < items4 && i < np) ? A.primes[i] : 0u;
+  };
+  uint32_t it4 = take4();
+  uint32_t f4 = first4(it4);
+  while (it4 < items4) {
+    const uint32_t nit4 = take4();
+    const uint32_t nf4 = first4(nit4);
+    const uint32_t b0 = n3 + 64 * g4 * it4;
+    const uint32_t b1 = min(np, b0 + 64 * g4);
+    uint32_t e_next = f4;
+    for (uint32_t b = b0; b < b1; b += 64) {
+      const uint32_t i = b + lane;
+      const uint32_t e4 = e_next;
+      e_next = (i + 64 < b1) ? A.primes[i + 64] : 0u;
+      if (i >= b1) continue;
+      const uint32_t p = e4 & 0xFFFFFFu, r = e4 >> 24;
+      const uint32_t rk = mod_u64(k0, p);
+      if (p >= plow) {  // the p*p rule can bite: general path
+        cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
+        continue;
       }
+      const uint32_t q30 = p / 30;
+      const uint2 fr = *reinterpret_cast<const uint2*>(tab + r * 8);
+      const uint2 br = *reinterpret_cast<const uint2*>(tab + 64 + r * 8);
+      auto cc = [&](uint32_t c) { return __umul24(q30, w30(c)) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
+      auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(br.y, br.x, c | 0x0C0C0C00u) & 7u; };
+      // c = number of classes with cc_c < rk.  cc_c = q30 W[c] + frac_c with
+      // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
+      // counts, every class with W > w does not, and only W == w needs a look.
+      uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
+      if ((uint32_t)__umul24(w, q30) > rk) --w;
+      if ((uint32_t)__umul24(w + 1, q30) <= rk) ++w;
+      const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
+      uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
+      if (c < 8 && cc(c) < rk) ++c;
+      const uint32_t end = rk + nbytes;
+      for (; c < 8; ++c) {
+        const uint32_t x = cc(c);
+        if (x >= end) break;
+        cross_one(base, x - rk, bit_of(c));
+      }
+      if (end > p) {
+        const uint32_t end2 = end - p;
+        for (uint32_t c2 = 0; c2 < 8; ++c2) {
+          const uint32_t x = cc(c2);
+          if (x >= end2) break;
+          cross_one(base, x + p - rk, bit_of(c2));
+        }
+      }
+
     }
+    it4 = nit4;
+    f4 = nf4;
   }
 }
 
