@@ -120,6 +120,7 @@ namespace {
 // misc word layout
 constexpr uint32_t kMiscItem = 0;       // current work item
 constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work items: tiers 0-3 (word 1), tier 4 (word 2)
+constexpr uint32_t kMiscSplit = 3;      // stats mode: checkpoint split bit, first and second bucket (words 3-5)
 constexpr uint32_t kMiscWave = 8;       // 8 words per wave below
 // per wave: [0] first, [1] last, [2] count, [3] c41, [4] c31, [5] maxgap,
 //           [6] lower count, [7] lower c41 ... (lower c31 in [kMiscWave + 8*NW])
@@ -922,6 +923,22 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       if (cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
       cur_query = g.query;
     }
+    if (MODE == kModeStats && threadIdx.x == 0) {
+      // checkpoint split: at most one bucket boundary per segment; computed
+      // once here (two 64-bit divisions) and read by every thread after the
+      // barriers below
+      uint32_t split = 0xFFFFFFFFu, b0 = 0, b1 = 0;
+      if (A.cp_interval) {
+        const uint64_t vfirst = 30 * g.k0 + 1;
+        const uint64_t vlast = 30 * (g.k0 + g.nbytes) - 1;
+        b0 = (uint32_t)((vfirst - 1) / A.cp_interval);
+        b1 = (uint32_t)((vlast - 1) / A.cp_interval);
+        if (b1 != b0) split = bit_at_or_above((uint64_t)(b0 + 1) * A.cp_interval + 1, g.k0, g.nbytes);
+      }
+      lds[kLdsMisc + kMiscSplit] = split;
+      lds[kLdsMisc + kMiscSplit + 1] = b0;
+      lds[kLdsMisc + kMiscSplit + 2] = b1;
+    }
     const bool cached = MODE == kModeBatch && g.from_cache;  // uniform per workgroup
     if (cached) {
       cache_segment<BS>(A, lds, g);
@@ -946,15 +963,11 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       continue;
     }
 
-    // checkpoint split (stats mode): at most one bucket boundary per segment
-    uint32_t split = 0xFFFFFFFFu;
-    uint32_t b0 = 0, b1 = 0;
-    if (MODE == kModeStats && A.cp_interval) {
-      const uint64_t vfirst = 30 * g.k0 + 1;
-      const uint64_t vlast = 30 * (g.k0 + g.nbytes) - 1;
-      b0 = (uint32_t)((vfirst - 1) / A.cp_interval);
-      b1 = (uint32_t)((vlast - 1) / A.cp_interval);
-      if (b1 != b0) split = bit_at_or_above((uint64_t)(b0 + 1) * A.cp_interval + 1, g.k0, g.nbytes);
+    uint32_t split = 0xFFFFFFFFu, b0 = 0, b1 = 0;
+    if (MODE == kModeStats) {
+      split = lds[kLdsMisc + kMiscSplit];
+      b0 = lds[kLdsMisc + kMiscSplit + 1];
+      b1 = lds[kLdsMisc + kMiscSplit + 2];
     }
     walk_segment<MODE, BS>(A, lds, g, split);
     __syncthreads();
