@@ -121,6 +121,7 @@ namespace {
 constexpr uint32_t kMiscItem = 0;       // current work item
 constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work items: tiers 0-3 (word 1), tier 4 (word 2)
 constexpr uint32_t kMiscSplit = 3;      // stats mode: checkpoint split bit, first and second bucket (words 3-5)
+constexpr uint32_t kMiscBounds = 160;   // segment tier bounds (6 words; above the per-wave records of 16 waves)
 constexpr uint32_t kMiscWave = 8;       // 8 words per wave below
 // per wave: [0] first, [1] last, [2] count, [3] c41, [4] c31, [5] maxgap,
 //           [6] lower count, [7] lower c41 ... (lower c31 in [kMiscWave + 8*NW])
@@ -164,6 +165,45 @@ __device__ void init_tables(uint32_t* lds) {
   }
 }
 
+// The sieving-prime index bounds of a segment's tiers and plow, the least
+// prime for which the p*p rule can bite (see sieve_segment).  Computed once
+// per segment by one thread in init_segment and read from LDS by all.
+struct SegBounds {
+  uint32_t np, n0, n1, n2, n3, plow;
+};
+__device__ __forceinline__ SegBounds segment_bounds(const SieveArgs& A, const SegGeom& g) {
+  const uint64_t k0 = g.k0;
+  const uint32_t nbytes = g.nbytes;
+  SegBounds B;
+  uint64_t top = 30 * (k0 + nbytes) - 1;
+  if (g.hi_val < top) top = g.hi_val;
+  const uint64_t sq = isqrt_u64(top);
+  uint64_t ti = sq / kPiTabStep + 1;
+  if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
+  const uint32_t np = B.np = A.pi_tab[ti];
+  const uint32_t n0 = B.n0 = min(np, A.tier0_end);
+  const uint32_t n1 = B.n1 = max(n0, min(np, A.tier1_end));
+  // the sparse tier is exact for every p >= nbytes, so a short segment (a
+  // small range query) moves it down: primes past the pi-table bin that
+  // holds nbytes all exceed nbytes
+  const uint32_t nshort = A.pi_tab[min((nbytes + kPiTabStep - 1) / kPiTabStep, A.pi_tab_len - 1)];
+  const uint32_t n3 = B.n3 = max(n1, min(np, min(A.tier3_end, nshort)));
+  B.n2 = max(n1, min(n3, A.tier2_end));
+  // the p*p rule can bite exactly for p >= plow, the least p with
+  // p * (p + 30) > 30 * k0 (one test per prime instead of a 64-bit product)
+  {
+    const uint64_t t30 = 30 * k0;
+    uint64_t x = (uint64_t)((__builtin_sqrt(900.0 + 4.0 * (double)t30) - 30.0) * 0.5);
+    while (x > 0 && (x - 1) * (x + 29) > t30) --x;
+    while (x * (x + 30) <= t30) ++x;
+    B.plow = (uint32_t)min(x, (uint64_t)0xFFFFFFFFu);
+  }
+
+  (void)n0;
+  (void)n3;
+  return B;
+}
+
 // Initialise the segment in LDS as the AND of the presieve patterns (primes
 // 7..19 removed), zero-filling up to the next 1 KiB so later passes read whole
 // 16-byte chunks.  Bytes 0..3 of the global bitmap (values below 120) get
@@ -176,6 +216,10 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   for (int k = 0; k < kPresieveGroups; ++k) off[k] = (uint32_t)(g.k0 % kPresievePeriods[k]);
   const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
   if (threadIdx.x < 2) lds[kLdsMisc + kMiscSieveCtr + threadIdx.x] = 0;  // sieve_segment's work counters
+  // thread 0 computes the tier bounds first, so their table loads overlap its
+  // pattern loads, and stores them after its chunks
+  SegBounds sb;
+  if (threadIdx.x == 0) sb = segment_bounds(A, g);
   for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
     uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
@@ -206,6 +250,10 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
       v.x = w[0]; v.y = w[1]; v.z = w[2]; v.w = w[3];
     }
     *reinterpret_cast<uint4*>(seg8 + b0) = v;
+  }
+  if (threadIdx.x == 0) {
+    uint32_t* gb = lds + kLdsMisc + kMiscBounds;
+    gb[0] = sb.np; gb[1] = sb.n0; gb[2] = sb.n1; gb[3] = sb.n2; gb[4] = sb.n3; gb[5] = sb.plow;
   }
 }
 
@@ -391,36 +439,14 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
 template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63;
   const uint8_t* tab = reinterpret_cast<const uint8_t*>(lds + kLdsTab);
   const uint32_t base = seg_base_addr(lds);
   const uint64_t k0 = g.k0;
   const uint32_t nbytes = g.nbytes;
 
-  uint64_t top = 30 * (k0 + nbytes) - 1;
-  if (g.hi_val < top) top = g.hi_val;
-  const uint64_t sq = isqrt_u64(top);
-  uint64_t ti = sq / kPiTabStep + 1;
-  if (ti >= A.pi_tab_len) ti = A.pi_tab_len - 1;
-  const uint32_t np = A.pi_tab[ti];
-  const uint32_t n0 = min(np, A.tier0_end);
-  const uint32_t n1 = max(n0, min(np, A.tier1_end));
-  // the sparse tier is exact for every p >= nbytes, so a short segment (a
-  // small range query) moves it down: primes past the pi-table bin that
-  // holds nbytes all exceed nbytes
-  const uint32_t nshort = A.pi_tab[min((nbytes + kPiTabStep - 1) / kPiTabStep, A.pi_tab_len - 1)];
-  const uint32_t n3 = max(n1, min(np, min(A.tier3_end, nshort)));
-  const uint32_t n2 = max(n1, min(n3, A.tier2_end));
-  // the p*p rule can bite exactly for p >= plow, the least p with
-  // p * (p + 30) > 30 * k0 (one test per prime instead of a 64-bit product)
-  uint32_t plow;
-  {
-    const uint64_t t30 = 30 * k0;
-    uint64_t x = (uint64_t)((__builtin_sqrt(900.0 + 4.0 * (double)t30) - 30.0) * 0.5);
-    while (x > 0 && (x - 1) * (x + 29) > t30) --x;
-    while (x * (x + 30) <= t30) ++x;
-    plow = (uint32_t)min(x, (uint64_t)0xFFFFFFFFu);
-  }
+  const uint32_t* gb = lds + kLdsMisc + kMiscBounds;
+  const uint32_t np = gb[0], n0 = gb[1], n1 = gb[2], n2 = gb[3], n3 = gb[4], plow = gb[5];
 
   // Work items of tiers 0-3, heaviest first: the 64-prime chunks of tier 2
   // (its first chunk alone is a large share of a wave's work), tier-0
