@@ -75,6 +75,7 @@ struct SieveArgs {
   uint32_t pi_tab_len;
   const uint32_t* presieve;     // pattern bytes as u32 words, kPresieveGroups patterns
   uint32_t presieve_off[kPresieveGroups];  // word offset of each pattern
+  uint32_t presieve_copy_words;  // pattern 0 is stored 4 times, copy c shifted by c bytes, this many words apart
   uint32_t first_word;          // true prime bitmap of bytes 0..3 (values < 120)
   // geometry
   uint64_t k_base;              // sweep: byte index of segment 0

@@ -211,6 +211,7 @@ class Engine {
   uint32_t first_word_ = 0;
   uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
   uint32_t tier0_p_ = kTier0P;  // primes below: kernel tier 0
+  uint32_t presieve_copy_words_ = 0;
   uint32_t tier2_div_ = 4;      // primes below seg_bytes / this: kernel tier 2
   KernelTimes times_;
 };

@@ -121,8 +121,8 @@ Engine::Engine() {
   for (int gi = 0; gi < kPresieveGroups; ++gi) {
     presieve_off_[gi] = (uint32_t)words.size();
     const size_t len = ((size_t)kPresievePeriods[gi] + kMaxSegBytes + 4096 + 3) & ~(size_t)3;
-    std::vector<uint8_t> pat(len);
-    for (size_t j = 0; j < len; ++j) {
+    std::vector<uint8_t> pat(len + 4);
+    for (size_t j = 0; j < len + 4; ++j) {
       uint8_t b = 0;
       for (int o = 0; o < 8; ++o) {
         const uint64_t v = 30ull * j + kW30[o];
@@ -132,9 +132,14 @@ Engine::Engine() {
       }
       pat[j] = b;
     }
-    const size_t w0 = words.size();
-    words.resize(w0 + len / 4);
-    std::memcpy(words.data() + w0, pat.data(), len);
+    // four copies, copy c starting c bytes in, so a read at any byte phase
+    // starts on a word of one of them (the kernel copies it to LDS directly)
+    presieve_copy_words_ = (uint32_t)(len / 4);
+    for (int c = 0; c < 4; ++c) {
+      const size_t w0 = words.size();
+      words.resize(w0 + len / 4);
+      std::memcpy(words.data() + w0, pat.data() + c, len);
+    }
   }
   hip_check(hipMalloc(&d_presieve_, words.size() * 4), "hipMalloc(presieve)");
   hip_check(hipMemcpy(d_presieve_, words.data(), words.size() * 4, hipMemcpyHostToDevice), "hipMemcpy(presieve)");
@@ -256,6 +261,7 @@ SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   A.pi_tab_len = pitab_len_;
   A.presieve = d_presieve_;
   for (int k = 0; k < kPresieveGroups; ++k) A.presieve_off[k] = presieve_off_[k];
+  A.presieve_copy_words = presieve_copy_words_;
   A.first_word = first_word_;
   A.seg_bytes = c.seg_bytes;
   A.nseg = nseg;

@@ -220,19 +220,29 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   // pattern loads, and stores them after its chunks
   SegBounds sb;
   if (threadIdx.x == 0) sb = segment_bounds(A, g);
-  for (uint32_t t = threadIdx.x; t < nchunks; t += BS) {
-    uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
-#pragma unroll
-    for (int k = 0; k < kPresieveGroups; ++k) {
-      const uint32_t src = off[k] + 16 * t;
-      const uint32_t* p = A.presieve + A.presieve_off[k] + (src >> 2);
-      const uint32_t sh = src & 3;
-      const uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
-      v.x &= __builtin_amdgcn_alignbyte(a1, a0, sh);
-      v.y &= __builtin_amdgcn_alignbyte(a2, a1, sh);
-      v.z &= __builtin_amdgcn_alignbyte(a3, a2, sh);
-      v.w &= __builtin_amdgcn_alignbyte(a4, a3, sh);
-    }
+  // Whole 256-byte blocks go straight from the pattern to LDS
+  // (global_load_lds: no registers, every block of a wave in flight at
+  // once), from the copy whose byte phase makes the source word-aligned.
+  // Threads fill the rest: block 0 of segment 0 (its true bits patched in),
+  // the partial last block and the zero fill up to the next 1 KiB.
+  static_assert(kPresieveGroups == 1, "one presieve pattern");
+  const uint32_t ph = off[0] & 3;
+  const uint32_t* src = A.presieve + A.presieve_off[0] + ph * A.presieve_copy_words + ((off[0] - ph) >> 2);
+  const uint32_t nfull = g.nbytes >> 8;
+  const uint32_t first = (g.k0 < 4 && nfull > 0) ? 1u : 0u;  // else the thread pass covers chunk 0
+  {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t blk = first + wave; blk < nfull; blk += BS / 64)
+      __builtin_amdgcn_global_load_lds((const void*)(src + 64 * blk + lane),
+                                       (__attribute__((address_space(3))) void*)((lds_u32*)(lds + kLdsSeg) + 64 * blk),
+                                       4, 0, 0);
+  }
+  const uint32_t nhead = first ? 16u : 0u;                 // 16-byte chunks of block 0
+  const uint32_t nrest = nhead + (nchunks - 16 * nfull);   // plus the tail chunks
+  for (uint32_t r = threadIdx.x; r < nrest; r += BS) {
+    const uint32_t t = r < nhead ? r : 16 * nfull + (r - nhead);
+    uint4 v;
+    v.x = src[4 * t]; v.y = src[4 * t + 1]; v.z = src[4 * t + 2]; v.w = src[4 * t + 3];
     if (g.k0 < 4 && t == 0) {  // global bytes k0..3 hold the presieved primes themselves
       const uint32_t sh = 8 * (uint32_t)g.k0;
       const uint32_t m = 0xFFFFFFFFu >> sh;
@@ -251,6 +261,7 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     }
     *reinterpret_cast<uint4*>(seg8 + b0) = v;
   }
+  __builtin_amdgcn_s_waitcnt(0);  // the direct LDS loads are counted as vector memory loads
   if (threadIdx.x == 0) {
     uint32_t* gb = lds + kLdsMisc + kMiscBounds;
     gb[0] = sb.np; gb[1] = sb.n0; gb[2] = sb.n1; gb[3] = sb.n2; gb[4] = sb.n3; gb[5] = sb.plow;
