@@ -791,7 +791,16 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
   const uint32_t wbeg = wave * wave_bytes;
   const uint32_t wend = min(wbeg + wave_bytes, g.nbytes);
+  // waves further along drop in issue priority, so lagging waves catch up
+  // and the walk's waves reach the barrier together
+  __builtin_amdgcn_s_setprio(3);
   for (uint32_t base = wbeg; base < wend; base += 1024) {
+    {
+      const uint32_t done = (base - wbeg) >> 10;
+      if (done == 3) __builtin_amdgcn_s_setprio(2);
+      else if (done == 6) __builtin_amdgcn_s_setprio(1);
+      else if (done == 8) __builtin_amdgcn_s_setprio(0);
+    }
     const uint32_t boff = base + 16 * lane;
     uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
     if (partial) {
@@ -844,6 +853,7 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     if (((q.x | q.y | q.z | q.w) != 0) && lm < 28) maxg = max(maxg, 28u);
     if (lm >= slow) chunk_big_gaps<MODE>(A, lds, q, boff, prev0, g, copy);
   }
+  __builtin_amdgcn_s_setprio(0);
   // per-wave reductions
   cnt = wave_sum(cnt);
   if (MODE == kModeStats) {
