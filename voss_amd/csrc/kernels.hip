@@ -204,10 +204,11 @@ __device__ __forceinline__ SegBounds segment_bounds(const SieveArgs& A, const Se
   return B;
 }
 
-// Initialise the segment in LDS as the AND of the presieve patterns (primes
+// Initialise the segment in LDS from the presieve pattern (multiples of
 // 7..19 removed), zero-filling up to the next 1 KiB so later passes read whole
 // 16-byte chunks.  Bytes 0..3 of the global bitmap (values below 120) get
-// their true bits: 1 is not prime, and 7..113 must stay set.
+// their true bits: 1 is not prime, and 7..113 must stay set.  Also clears the
+// crossing-off work counters and publishes the segment's tier bounds.
 template <int BS>
 __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
