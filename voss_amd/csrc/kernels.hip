@@ -233,10 +233,12 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   const uint32_t first = (g.k0 < 4 && nfull > 0) ? 1u : 0u;  // else the thread pass covers chunk 0
   {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t blk = first + wave; blk < nfull; blk += BS / 64)
-      __builtin_amdgcn_global_load_lds((const void*)(src + 64 * blk + lane),
-                                       (__attribute__((address_space(3))) void*)((lds_u32*)(lds + kLdsSeg) + 64 * blk),
-                                       4, 0, 0);
+    // wave 0 may still be merging the previous segment: waves 1.. copy
+    if (wave > 0)
+      for (uint32_t blk = first + wave - 1; blk < nfull; blk += BS / 64 - 1)
+        __builtin_amdgcn_global_load_lds((const void*)(src + 64 * blk + lane),
+                                         (__attribute__((address_space(3))) void*)((lds_u32*)(lds + kLdsSeg) + 64 * blk),
+                                         4, 0, 0);
   }
   const uint32_t nhead = first ? 16u : 0u;                 // 16-byte chunks of block 0
   const uint32_t nrest = nhead + (nchunks - 16 * nfull);   // plus the tail chunks
@@ -1094,8 +1096,13 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
         if (dst) atomicAdd(dst, (unsigned long long)val);
       }
     }
-    __syncthreads();
+    // Sweeps need no barrier here: the next segment's init touches nothing
+    // the merge reads, so waves 1.. start it (and its pattern copies, which
+    // wave 0 leaves to them) while wave 0 merges, and the init barrier joins
+    // them.  The histogram flush reads what the merge adds: it waits first.
+    if (MODE == kModeBatch) __syncthreads();
     if (MODE == kModeStats && ++since_flush == 1024) {
+      __syncthreads();
       flush_hist_stats<BS>(A, lds);
       since_flush = 0;
       __syncthreads();
@@ -1108,7 +1115,10 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       s += gridDim.x;
     }
   }
-  if (MODE == kModeStats) flush_hist_stats<BS>(A, lds);
+  if (MODE == kModeStats) {
+    __syncthreads();
+    flush_hist_stats<BS>(A, lds);
+  }
   if (MODE == kModeBatch && cur_query != 0xFFFFFFFFu) flush_hist_query<BS>(A, lds, cur_query);
 }
 
