@@ -1127,9 +1127,45 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
 // to the histogram and large-gap list; boundary records are emitted
 // directly; segments whose internal largest gap beats every earlier gap are
 // listed for rescan_kernel together with the running maximum before them.
+// Exclusive scan of one value per thread over a 1024-thread block, in thread
+// order, under an associative op(earlier, later) with identity `ident`:
+// 64-lane shuffle scans, then a scan of the 16 wave totals.  Values are
+// 64-bit (two 32-bit shuffles each step).  `tot` is 32 words of shared
+// scratch; every thread must call it (it has barriers).
+template <typename Op>
+__device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t ident, Op op, unsigned long long* tot) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  auto up = [](uint64_t x, int d) -> uint64_t {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)x, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(x >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+  };
+  uint64_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint64_t u = up(incl, d);
+    if (lane >= (uint32_t)d) incl = op(u, incl);
+  }
+  uint64_t excl = up(incl, 1);
+  if (lane == 0) excl = ident;
+  if (lane == 63) tot[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    uint64_t w = lane < 16 ? (uint64_t)tot[lane] : ident;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+      const uint64_t u = up(w, d);
+      if (lane >= (uint32_t)d) w = op(u, w);
+    }
+    if (lane < 16) tot[16 + lane] = w;
+  }
+  __syncthreads();
+  const uint64_t before = wave ? (uint64_t)tot[16 + wave - 1] : ident;
+  return op(before, excl);
+}
+
 __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
-  __shared__ unsigned long long s_last[1024];
-  __shared__ uint32_t s_max[1024];
+  __shared__ unsigned long long s_tot[32];
   __shared__ uint32_t s_hist[kHistBins];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < kHistBins; i += 1024) s_hist[i] = 0;
@@ -1148,18 +1184,10 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
     if (last) emax = max(emax, (uint32_t)(F - last));
     last = absval(s, S.last);
   }
-  s_last[t] = last;
-  __syncthreads();
-  if (t == 0) {  // exclusive running "last prime before" across threads
-    unsigned long long run = A.seed_last;
-    for (int i = 0; i < 1024; ++i) {
-      unsigned long long v = s_last[i];
-      s_last[i] = run;
-      if (v) run = v;
-    }
-  }
-  __syncthreads();
-  const uint64_t prev0 = s_last[t];
+  // the last prime before this thread's segments: the last non-zero "last"
+  // of the threads before it, else the seed
+  const uint64_t lastb = block_exclusive_scan(last, 0, [](uint64_t a, uint64_t b) { return b ? b : a; }, s_tot);
+  const uint64_t prev0 = lastb ? lastb : (uint64_t)A.seed_last;
   // boundary gap of this thread's first non-empty segment
   for (uint32_t s = lo; s < hi; ++s) {
     const SegSummary S = A.summaries[s];
@@ -1167,18 +1195,10 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
     emax = max(emax, (uint32_t)(absval(s, S.first) - prev0));
     break;
   }
-  s_max[t] = emax;
-  __syncthreads();
-  if (t == 0) {
-    uint32_t run = A.seed_max;
-    for (int i = 0; i < 1024; ++i) {
-      uint32_t v = s_max[i];
-      s_max[i] = run;
-      run = max(run, v);
-    }
-  }
-  __syncthreads();
-  uint32_t run = s_max[t];
+  __syncthreads();  // s_tot is reused
+  // the largest gap before this thread's segments
+  uint32_t run = (uint32_t)block_exclusive_scan(emax, 0, [](uint64_t a, uint64_t b) { return a > b ? a : b; }, s_tot);
+  run = max(run, (uint32_t)A.seed_max);
   uint64_t prev = prev0;
   for (uint32_t s = lo; s < hi; ++s) {
     const SegSummary S = A.summaries[s];
