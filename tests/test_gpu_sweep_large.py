@@ -31,6 +31,32 @@ def test_full_stats_1e11_vs_oracle(voss, oracle):
     assert g["records"][-1] == (42_652_618_343 + 464, 464)
 
 
+def test_context_progress_splits_1e11(voss):
+    # a progress callback splits the sweep into launches of about 5e10
+    # integers each (context.cu:260-269): at 1e11 that is two full launches
+    # and a last one of a few segments.  Every Context result must equal the
+    # single-launch sweep's and the published values.
+    N = 10**11
+
+    def results(c):
+        # the result classes compare by identity, as the reference's do
+        cb, st = c.chebyshev(), c.statistics()
+        return (c.prime_count(), c.twins(), c.cousin(), c.sexy(), c.sophie_germain(), cb.pi_4_1, cb.pi_4_3,
+                st.mean_gap, st.std_dev, st.skewness, st.kurtosis, st.total_gaps, c.large_gaps())
+
+    seen = []
+    with voss.primes.Context(N, profile="full") as c:
+        c.set_progress(lambda *a: seen.append(a))
+        split = results(c)
+    with voss.primes.Context(N, profile="full") as c:
+        whole = results(c)
+    assert [a[0] for a in seen] == [1, 2, 3] and all(a[1] == 3 for a in seen)
+    assert sum(a[2] for a in seen) == 4_118_054_813 - 3 and seen[-1][3] == 4_118_054_813
+    assert seen[0][3] > 1_900_000_000 and seen[1][3] == 4_118_054_813 - seen[2][2]
+    assert split == whole
+    assert split[0] == 4_118_054_813 and split[1] == 224_376_048
+
+
 def _fixture():
     if not FIX.exists():
         pytest.skip("tests/golden/sweep_1e12.json not written")

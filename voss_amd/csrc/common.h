@@ -43,11 +43,15 @@ struct SegSummary {
   uint32_t count;   // primes in the segment
 };
 
-// A flagged segment whose records must be listed: every gap > threshold in
-// order, each raising the threshold.
+// A flagged wave range (the seg_bytes / waves bytes one wave of the sweep
+// walked) whose records must be listed: every gap > threshold in order, each
+// raising the threshold.  dprev = distance from the last prime before the
+// range (in the same segment) to the range's first value, 0 if none.
 struct RescanItem {
   uint32_t seg;
   uint32_t threshold;
+  uint32_t wave;
+  uint32_t dprev;
 };
 
 // Batched range query, flattened into segments by the host.
@@ -93,6 +97,7 @@ struct SieveArgs {
   // outputs (mode dependent; unused ones may be null)
   unsigned long long* totals;   // [0] count, [1] pi_4_1 part, [2] pi_3_1 part
   SegSummary* summaries;        // indexed by global segment
+  uint32_t* wave_gaps;          // stats mode: per global segment and wave, (dprev << 16) | largest gap
   unsigned long long* hist;     // kGapBins u64
   unsigned long long* buckets;  // per checkpoint bucket: count, c41, c31
   uint64_t cp_interval;         // 0 = no buckets

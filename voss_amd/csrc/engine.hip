@@ -399,7 +399,9 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   }
   const uint32_t nlaunch = (nseg + per_launch - 1) / per_launch;
 
+  const uint32_t waves = (uint32_t)cfg_.block / 64;
   DevBuf& d_sum = ws(kWsSummaries, (size_t)nseg * sizeof(SegSummary));
+  DevBuf& d_wg = ws(kWsWaveGaps, (size_t)nseg * waves * 4);
   DevBuf& d_tot = ws(kWsTotals, 64);
   DevBuf& d_hist = ws(kWsHist, (size_t)kGapBins * 8);
   DevBuf& d_bk = ws(kWsBuckets, (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8);
@@ -423,6 +425,7 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   A.hi_val = N;
   A.totals = d_tot.as<unsigned long long>();
   A.summaries = d_sum.as<SegSummary>();
+  A.wave_gaps = d_wg.as<uint32_t>();
   A.hist = d_hist.as<unsigned long long>();
   A.buckets = d_bk.as<unsigned long long>();
   A.cp_interval = cp_interval;
@@ -455,6 +458,8 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   StitchArgs T;
   std::memset(&T, 0, sizeof(T));
   T.summaries = d_sum.as<SegSummary>();
+  T.wave_gaps = d_wg.as<uint32_t>();
+  T.waves = waves;
   T.nseg = nseg;
   T.seg_bytes = S;
   T.k_base = 0;
@@ -475,7 +480,7 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   hip_check(launch_stitch(T, stream_), "launch(stitch)");
   hip_check(hipEventRecord(e2.e, stream_), "hipEventRecord");
   {
-    // records of flagged segments: the item count stays on the device
+    // records of flagged wave ranges: the item count stays on the device
     RescanArgs RA;
     RA.items = d_resc.as<RescanItem>();
     RA.n_items = d_cnt.as<unsigned int>() + 2;

@@ -8,6 +8,8 @@ namespace voss {
 
 struct StitchArgs {
   const SegSummary* summaries;
+  const uint32_t* wave_gaps;  // per segment and wave: (dprev << 16) | largest gap
+  uint32_t waves;             // waves per segment of the sweep
   uint32_t nseg;
   uint32_t seg_bytes;
   uint64_t k_base;
@@ -55,7 +57,7 @@ hipError_t prepare_kernels(uint32_t lds_bytes);
 
 hipError_t launch_sieve(int mode, int block, const SieveArgs& A, int grid, hipStream_t st);
 hipError_t launch_stitch(const StitchArgs& A, hipStream_t st);
-// grid workgroups loop over the *R.n_items flagged segments
+// grid workgroups loop over the *R.n_items flagged wave ranges
 hipError_t launch_rescan(int block, const SieveArgs& A, const RescanArgs& R, int grid, hipStream_t st);
 hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStream_t st);
 // k-th prime (1-based ranks[i]) of sweep segment segs[i] for i < *n_dev; one

@@ -1046,6 +1046,14 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
         smax = mw[5];
         if (before) note_gap<MODE>(A, lds, wf - before, wf, g, 0, smax);
       }
+      if (MODE == kModeStats && A.wave_gaps && mine) {
+        // for the records rescan: the wave's largest gap (the gap into its
+        // first prime included) and the distance from the segment's last
+        // earlier prime to the wave's first value.  Both are below one gap
+        // between primes (< 1600 below 2^64), so 16 bits hold each.
+        const uint32_t d = before ? 30u * lane * (A.seg_bytes / NW) - before : 0u;
+        A.wave_gaps[(size_t)(A.seg_base + s) * NW + lane] = (min(d, 0xFFFFu) << 16) | min(smax, 0xFFFFu);
+      }
       const uint64_t hb = __ballot(has);
       const uint32_t sfirst = (uint32_t)__shfl((int)wf, hb ? (int)__builtin_ctzll(hb) : 0, 64);
       const uint32_t slast = (uint32_t)__shfl((int)incl, NW - 1, 64);
@@ -1125,8 +1133,9 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
 // ---------------------------------------------------------------- stitch (sweep)
 // One workgroup joins the per-segment summaries in order: boundary gaps go
 // to the histogram and large-gap list; boundary records are emitted
-// directly; segments whose internal largest gap beats every earlier gap are
-// listed for rescan_kernel together with the running maximum before them.
+// directly; within segments whose internal largest gap beats every earlier
+// gap, each wave range (the bytes one wave walked) that raises the running
+// maximum is listed for rescan_kernel with the maximum before it.
 // Exclusive scan of one value per thread over a 1024-thread block, in thread
 // order, under an associative op(earlier, later) with identity `ident`:
 // 64-lane shuffle scans, then a scan of the 16 wave totals.  Values are
@@ -1218,9 +1227,16 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
       run = B;
     }
     if (S.maxgap > run) {
-      unsigned int slot = atomicAdd(A.n_rescan, 1u);
-      if (slot < A.rescan_cap) { A.rescan[slot].seg = s; A.rescan[slot].threshold = run; }
-      run = S.maxgap;
+      // list each wave range of the segment that raises the running maximum
+      const uint32_t* wg = A.wave_gaps + (size_t)s * A.waves;
+      for (uint32_t w = 0; w < A.waves; ++w) {
+        const uint32_t x = wg[w];
+        if ((x & 0xFFFFu) <= run) continue;
+        unsigned int slot = atomicAdd(A.n_rescan, 1u);
+        if (slot < A.rescan_cap) A.rescan[slot] = RescanItem{s, run, w, x >> 16};
+        run = x & 0xFFFFu;
+      }
+      run = max(run, S.maxgap);
     }
     prev = absval(s, S.last);
   }
@@ -1234,17 +1250,18 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
 }
 
 // ---------------------------------------------------------------- records rescan
-// One workgroup per flagged segment: re-sieve it and list, in one ordered
-// pass, every gap larger than all earlier gaps (starting from the running
-// maximum before the segment).  Pass 1 takes each (wave, step)'s largest
-// gap; thread 0 turns those into exclusive prefix maxima in number order
-// (and checks the gaps between waves); pass 2 re-walks each chunk against
-// its prefix maximum and emits the records.
+// One workgroup per flagged wave range (the seg_bytes / waves bytes that one
+// wave of the sweep walked): re-sieve just that range, then wave 0 walks it
+// in order, 1 KiB per step, and lists every gap larger than all earlier gaps,
+// starting from the running maximum before the range and from the last prime
+// before it.  Each step takes the lanes' largest gaps, an exclusive prefix
+// maximum over the lanes, and lets the lanes whose chunk beats it emit.
 
-// Largest gap in a lane's 16-byte chunk (prev = last prime before it, 0 if
-// none), optionally emitting gaps above `run` in order.
+// Largest gap in a lane's 16-byte chunk at segment-relative byte boff (prev =
+// last prime before it, 0 if none), optionally emitting the gaps above `run`
+// in order.
 template <bool EMIT>
-__device__ __forceinline__ uint32_t chunk_gaps(uint4 q, uint32_t boff, uint32_t prev, uint32_t run, uint32_t* wfirst,
+__device__ __forceinline__ uint32_t chunk_gaps(uint4 q, uint32_t boff, uint32_t prev, uint32_t run,
                                                const RescanArgs& R, uint64_t k0, uint32_t seg) {
   uint32_t lm = 0;
   const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
@@ -1266,8 +1283,6 @@ __device__ __forceinline__ uint32_t chunk_gaps(uint4 q, uint32_t boff, uint32_t 
             R.out[slot].seg = seg;
           }
         }
-      } else if (wfirst) {
-        *wfirst = v;
       }
       prev = v;
     }
@@ -1283,104 +1298,56 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   init_tables(lds);
   __syncthreads();
   const uint32_t nitems = min(*R.n_items, R.items_cap);  // same value in every thread
+  const uint32_t sub = A.seg_bytes / NW;                  // bytes of one wave range (a multiple of 1 KiB)
   for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-  const RescanItem item = R.items[it];
-  const uint32_t s = item.seg;
-  SegGeom g;
-  g.query = 0;
-  g.from_cache = 0;
-  g.k0 = A.k_base + (uint64_t)s * A.seg_bytes;
-  const uint64_t rem = A.k_end - g.k0;
-  g.nbytes = rem < A.seg_bytes ? (uint32_t)rem : A.seg_bytes;
-  g.lo_val = A.lo_val;
-  g.hi_val = A.hi_val;
-  g.vlo = bit_at_or_above(g.lo_val, g.k0, g.nbytes);
-  g.vhi = bit_at_or_above(g.hi_val + 1, g.k0, g.nbytes);
-  init_segment<BS>(A, lds, g);
-  __syncthreads();
-  sieve_segment<BS>(A, lds, g);
-  __syncthreads();
+    const RescanItem item = R.items[it];
+    const uint32_t s = item.seg, off = item.wave * sub;
+    const uint64_t kseg = A.k_base + (uint64_t)s * A.seg_bytes;
+    SegGeom g;
+    g.query = 0;
+    g.from_cache = 0;
+    g.k0 = kseg + off;
+    const uint64_t rem = A.k_end - g.k0;  // > 0: the range holds primes
+    g.nbytes = rem < sub ? (uint32_t)rem : sub;
+    g.lo_val = A.lo_val;
+    g.hi_val = A.hi_val;
+    g.vlo = bit_at_or_above(g.lo_val, g.k0, g.nbytes);
+    g.vhi = bit_at_or_above(g.hi_val + 1, g.k0, g.nbytes);
+    init_segment<BS>(A, lds, g);
+    __syncthreads();
+    sieve_segment<BS>(A, lds, g);
+    __syncthreads();
 
-  const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
-  const uint32_t wave_bytes = A.seg_bytes / NW;
-  const uint32_t steps = wave_bytes / 1024;  // <= 18
-  uint32_t* stepmax = lds + kLdsMisc + 16;   // NW * steps <= 144 words
-  uint32_t* wfl = lds + kLdsMisc + 176;      // 2 words per wave
-  const uint32_t wbeg = wave * wave_bytes;
-
-  // pass 1: largest gap per (wave, step), first/last prime per wave
-  uint32_t carry = 0, wfirst = 0;
-  for (uint32_t st = 0; st < steps; ++st) {
-    const uint32_t boff = wbeg + 1024 * st + 16 * lane;
-    uint32_t lm = 0;
-    if (wbeg + 1024 * st < g.nbytes) {
-      uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
-      const uint32_t cb = 8 * boff;
-      if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
-      uint32_t step_last;
-      const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
-      carry = max(carry, step_last);
-      lm = chunk_gaps<false>(q, boff, prev, 0, &wfirst, R, g.k0, s);
-    }
-    lm = wave_max(lm);
-    if (lane == 0) stepmax[wave * steps + st] = lm;
-  }
-  wfirst = wave_max(wfirst);
-  if (lane == 0) { wfl[2 * wave] = wfirst; wfl[2 * wave + 1] = carry; }
-  __syncthreads();
-
-  // ordered prefix maxima; gaps between waves are checked here
-  if (threadIdx.x == 0) {
-    uint32_t run = item.threshold, last = 0;
-    for (int w = 0; w < NW; ++w) {
-      const uint32_t f = wfl[2 * w], l = wfl[2 * w + 1];
-      if (l && last) {
-        const uint32_t gap = f - last;
-        if (gap > run) {
-          run = gap;
-          const unsigned int slot = atomicAdd(R.n_out, 1u);
-          if (slot < R.out_cap) {
-            R.out[slot].position = 30 * g.k0 + f;
-            R.out[slot].gap = gap;
-            R.out[slot].seg = s;
-          }
-        }
-      }
-      for (uint32_t st = 0; st < steps; ++st) {
-        const uint32_t m = stepmax[w * steps + st];
-        stepmax[w * steps + st] = run;
-        run = max(run, m);
-      }
-      if (l) last = l;
-    }
-  }
-  __syncthreads();
-
-  // pass 2: emit the records of each chunk against its prefix maximum
-  carry = 0;
-  for (uint32_t st = 0; st < steps; ++st) {
-    if (wbeg + 1024 * st >= g.nbytes) break;  // wave-uniform
-    const uint32_t boff = wbeg + 1024 * st + 16 * lane;
-    uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
-    const uint32_t cb = 8 * boff;
-    if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
-    uint32_t step_last;
-    const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
-    carry = max(carry, step_last);
-    const uint32_t lm = chunk_gaps<false>(q, boff, prev, 0, nullptr, R, g.k0, s);
-    // exclusive max of lm over earlier lanes of this step
-    uint32_t incl = lm;
+    if (wave == 0) {
+      const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+      uint32_t run = item.threshold;
+      // values are segment-relative, as in the sweep
+      uint32_t carry = item.dprev ? 30 * off - item.dprev : 0;
+      for (uint32_t base = 0; base < g.nbytes; base += 1024) {
+        const uint32_t loc = base + 16 * lane;  // < sub: base is below nbytes <= sub
+        uint4 q = *reinterpret_cast<const uint4*>(seg8 + loc);
+        const uint32_t cb = 8 * loc;
+        if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+        const uint32_t boff = off + loc;
+        uint32_t step_last;
+        const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
+        carry = max(carry, step_last);
+        const uint32_t lm = chunk_gaps<false>(q, boff, prev, 0, R, kseg, s);
+        // exclusive max of lm over the earlier lanes of this step
+        uint32_t incl = lm;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
-      if (lane >= (uint32_t)d) incl = max(incl, t);
+        for (int d = 1; d < 64; d <<= 1) {
+          const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+          if (lane >= (uint32_t)d) incl = max(incl, t);
+        }
+        uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
+        if (lane == 0) excl = 0;
+        const uint32_t r = max(run, excl);
+        if (lm > r) chunk_gaps<true>(q, boff, prev, r, R, kseg, s);
+        run = max(run, (uint32_t)__shfl((int)incl, 63, 64));
+      }
     }
-    uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
-    if (lane == 0) excl = 0;
-    const uint32_t run = max(stepmax[wave * steps + st], excl);
-    if (lm > run) chunk_gaps<true>(q, boff, prev, run, nullptr, R, g.k0, s);
-  }
-  __syncthreads();  // the next item rebuilds the segment in LDS
+    __syncthreads();  // the next item rebuilds the range in LDS
   }
 }
 
