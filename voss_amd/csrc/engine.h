@@ -186,7 +186,7 @@ class Engine {
   // reusable device workspace, grown on demand
   enum WsSlot {
     kWsWork, kWsTotals, kWsSummaries, kWsHist, kWsBuckets, kWsLarge, kWsRec, kWsRescan,
-    kWsRescanOut, kWsCounters, kWsQueries, kWsSegQuery, kWsQHist, kWsQCount, kWsList, kWsWaveGaps, kWsCount_
+    kWsRescanOut, kWsCounters, kWsQueries, kWsSegQuery, kWsQHist, kWsQCount, kWsList, kWsWaveGaps, kWsStage, kWsCount_
   };
   DevBuf& ws(int slot, size_t bytes);
   std::vector<DevBuf> ws_;

@@ -412,11 +412,18 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   DevBuf& d_cnt = ws(kWsCounters, 64);  // [0] n_large [1] n_rec [2] n_rescan [3] n_rout, [8..] u64 final
   DevBuf& d_work = ws(kWsWork, (size_t)std::max<uint32_t>(nlaunch, 1) * 4 + 64);
 
-  hip_check(hipMemsetAsync(d_tot.p, 0, 64, stream_), "memset");
-  hip_check(hipMemsetAsync(d_hist.p, 0, (size_t)kGapBins * 8, stream_), "memset");
-  hip_check(hipMemsetAsync(d_bk.p, 0, (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8, stream_), "memset");
-  hip_check(hipMemsetAsync(d_cnt.p, 0, 64, stream_), "memset");
-  hip_check(hipMemsetAsync(d_work.p, 0, (size_t)std::max<uint32_t>(nlaunch, 1) * 4 + 64, stream_), "memset");
+  {
+    // every accumulator cleared by one launch
+    SpanList Z;
+    std::memset(&Z, 0, sizeof(Z));
+    auto zero = [&](DevBuf& b, size_t bytes) { Z.s[Z.n++] = CopySpan{nullptr, b.as<uint32_t>(), bytes / 4}; };
+    zero(d_tot, 64);
+    zero(d_hist, (size_t)kGapBins * 8);
+    zero(d_bk, (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8);
+    zero(d_cnt, 64);
+    zero(d_work, (size_t)std::max<uint32_t>(nlaunch, 1) * 4 + 64);
+    hip_check(launch_spans(Z, stream_), "launch(zero)");
+  }
 
   SieveArgs A = base_args(nseg);
   A.k_base = 0;
@@ -492,27 +499,38 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   }
   hip_check(hipEventRecord(e3.e, stream_), "hipEventRecord");
 
-  // one download into pinned staging: counters, totals, histogram, buckets
-  // and the first kHead entries of each list (the rest only if there are more)
-  constexpr uint32_t kHead = 4096;
+  // one gather launch packs counters, totals, the histogram's first
+  // kHistHead bins, buckets and the first kHead entries of each list into a
+  // staging block, and one copy downloads it (the rest only if needed)
+  constexpr uint32_t kHead = 1024;
+  constexpr uint32_t kHistHead = 2048;  // gaps below 4096
   const size_t o_cnt = 0, o_fin = 16, o_tot = 32, o_hist = 64;
-  const size_t o_bk = o_hist + (size_t)kGapBins * 8;
-  const size_t bk_bytes = (size_t)std::max<uint64_t>(nbuckets, 1) * 3 * 8;
+  const size_t o_bk = o_hist + (size_t)kHistHead * 8;
+  const size_t bk_bytes = (size_t)nbuckets * 3 * 8;
   const size_t o_lg = (o_bk + bk_bytes + 255) & ~(size_t)255;
   const size_t o_rc = o_lg + (size_t)kHead * sizeof(GapRec);
   const size_t o_ro = o_rc + (size_t)kHead * sizeof(GapRec);
-  uint8_t* hs = static_cast<uint8_t*>(pinned(o_ro + (size_t)kHead * sizeof(GapRec)));
-  auto d2h = [&](size_t off, const void* src, size_t n) {
-    hip_check(hipMemcpyAsync(hs + off, src, n, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
-  };
-  d2h(o_cnt, d_cnt.p, 16);
-  d2h(o_fin, d_cnt.as<unsigned long long>() + 4, 16);
-  d2h(o_tot, d_tot.p, 24);
-  d2h(o_hist, d_hist.p, (size_t)kGapBins * 8);
-  if (nbuckets) d2h(o_bk, d_bk.p, (size_t)nbuckets * 3 * 8);
-  d2h(o_lg, d_large.p, (size_t)kHead * sizeof(GapRec));
-  d2h(o_rc, d_rec.p, (size_t)kHead * sizeof(GapRec));
-  d2h(o_ro, d_rout.p, (size_t)kHead * sizeof(GapRec));
+  const size_t stage_bytes = o_ro + (size_t)kHead * sizeof(GapRec);
+  DevBuf& d_stage = ws(kWsStage, stage_bytes);
+  {
+    SpanList G;
+    std::memset(&G, 0, sizeof(G));
+    auto gather = [&](size_t off, const void* src, size_t bytes) {
+      G.s[G.n++] = CopySpan{static_cast<const uint32_t*>(src), reinterpret_cast<uint32_t*>(d_stage.as<uint8_t>() + off),
+                            bytes / 4};
+    };
+    gather(o_cnt, d_cnt.p, 16);
+    gather(o_fin, d_cnt.as<unsigned long long>() + 4, 16);
+    gather(o_tot, d_tot.p, 24);
+    gather(o_hist, d_hist.p, (size_t)kHistHead * 8);
+    if (nbuckets) gather(o_bk, d_bk.p, bk_bytes);
+    gather(o_lg, d_large.p, (size_t)kHead * sizeof(GapRec));
+    gather(o_rc, d_rec.p, (size_t)kHead * sizeof(GapRec));
+    gather(o_ro, d_rout.p, (size_t)kHead * sizeof(GapRec));
+    hip_check(launch_spans(G, stream_), "launch(gather)");
+  }
+  uint8_t* hs = static_cast<uint8_t*>(pinned(stage_bytes));
+  hip_check(hipMemcpyAsync(hs, d_stage.p, stage_bytes, hipMemcpyDeviceToHost, stream_), "hipMemcpy");
   hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   const unsigned int* cnts = reinterpret_cast<const unsigned int*>(hs + o_cnt);
   const unsigned long long* fin = reinterpret_cast<const unsigned long long*>(hs + o_fin);
@@ -544,7 +562,15 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   // ---- assemble (reference conventions)
   const uint64_t cnt = tot[0];
   R.prime_count = nseed + cnt;
-  for (uint32_t h = 1; h < kGapBins; ++h) R.hist[2 * h] += dh[h];
+  // bins above the largest gap's (fin[1] bounds every gap) are zero; they
+  // are downloaded only if that bound reaches past the staged ones
+  for (uint32_t h = 1; h < kHistHead; ++h) R.hist[2 * h] += dh[h];
+  if ((fin[1] >> 1) >= kHistHead) {
+    std::vector<unsigned long long> rest(kGapBins - kHistHead);
+    hip_check(hipMemcpy(rest.data(), d_hist.as<unsigned long long>() + kHistHead, rest.size() * 8,
+                        hipMemcpyDeviceToHost), "hipMemcpy");
+    for (uint32_t h = kHistHead; h < kGapBins; ++h) R.hist[2 * h] += rest[h - kHistHead];
+  }
   R.hist[1] += 1;  // 2 -> 3
   R.hist[2] += 1;  // 3 -> 5
   R.pi_4_1 = tot[1] + 1;
@@ -552,7 +578,6 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   R.pi_3_1 = tot[2];
   R.pi_3_2 = cnt - tot[2] + 2;
   R.last_prime = fin[0];
-  (void)fin[1];
   for (uint64_t g2 = R.hist.size(); g2-- > 0;)
     if (R.hist[g2]) { R.max_gap = g2; break; }
   if (ncp) {

@@ -67,6 +67,21 @@ hipError_t launch_select(int block, const SieveArgs& A, const uint32_t* segs, co
 // Zero n 32-bit words (graph-safe fill, no memset node).
 hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st);
 
+// Up to kMaxSpans word ranges zeroed (src null) or copied, in one launch: the
+// sweep clears its counters and packs its results for one download this way
+// instead of a memset or copy per buffer.
+constexpr int kMaxSpans = 8;
+struct CopySpan {
+  const uint32_t* src;  // null: zero dst
+  uint32_t* dst;
+  uint64_t words;
+};
+struct SpanList {
+  CopySpan s[kMaxSpans];
+  uint32_t n;
+};
+hipError_t launch_spans(const SpanList& L, hipStream_t st);
+
 // Bitmap -> ascending list of primes, in blocks of 4096 bitmap bytes:
 // counts[b] = primes in block b; out[block_off[b] + i] = i-th prime of block b.
 constexpr uint32_t kListBlockBytes = 4096;

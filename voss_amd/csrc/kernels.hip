@@ -850,6 +850,9 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
       wfirst = vfirst;
     }
     lm >>= 3;
+    // a chunk wholly past the last counted value has no prime to close its
+    // zero run: its distance from the last prime bounds no gap
+    if (partial && 8 * boff >= g.vhi) lm = 0;
     // internal gaps are at most 28; count them in the maximum only coarsely
     // (an over-estimate only triggers a records rescan of tiny segments)
     maxg = max(maxg, lm);
@@ -1183,35 +1186,41 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
   auto absval = [&](uint32_t s, uint32_t rel) -> uint64_t {
     return 30ull * (A.k_base + (uint64_t)s * A.seg_bytes) + rel;
   };
-  uint64_t last = 0;
+  // the thread's summaries in groups of 8, every load of a group issued
+  // before the first is used (one memory latency per group, not per segment)
+  auto for_each_summary = [&](auto&& f) {
+    for (uint32_t s0 = lo; s0 < hi; s0 += 8) {
+      SegSummary b[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = A.summaries[min(s0 + j, hi - 1)];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (s0 + j < hi) f(s0 + j, b[j]);
+    }
+  };
+  uint64_t last = 0, ffirst = 0;
   uint32_t emax = 0;
-  for (uint32_t s = lo; s < hi; ++s) {
-    const SegSummary S = A.summaries[s];
+  for_each_summary([&](uint32_t s, const SegSummary& S) {
     emax = max(emax, S.maxgap);
-    if (!S.count) continue;
+    if (!S.count) return;
     const uint64_t F = absval(s, S.first);
     if (last) emax = max(emax, (uint32_t)(F - last));
+    else ffirst = F;
     last = absval(s, S.last);
-  }
+  });
   // the last prime before this thread's segments: the last non-zero "last"
   // of the threads before it, else the seed
   const uint64_t lastb = block_exclusive_scan(last, 0, [](uint64_t a, uint64_t b) { return b ? b : a; }, s_tot);
   const uint64_t prev0 = lastb ? lastb : (uint64_t)A.seed_last;
   // boundary gap of this thread's first non-empty segment
-  for (uint32_t s = lo; s < hi; ++s) {
-    const SegSummary S = A.summaries[s];
-    if (!S.count) continue;
-    emax = max(emax, (uint32_t)(absval(s, S.first) - prev0));
-    break;
-  }
+  if (ffirst) emax = max(emax, (uint32_t)(ffirst - prev0));
   __syncthreads();  // s_tot is reused
   // the largest gap before this thread's segments
   uint32_t run = (uint32_t)block_exclusive_scan(emax, 0, [](uint64_t a, uint64_t b) { return a > b ? a : b; }, s_tot);
   run = max(run, (uint32_t)A.seed_max);
   uint64_t prev = prev0;
-  for (uint32_t s = lo; s < hi; ++s) {
-    const SegSummary S = A.summaries[s];
-    if (!S.count) continue;
+  for_each_summary([&](uint32_t s, const SegSummary& S) {
+    if (!S.count) return;
     const uint64_t F = absval(s, S.first);
     const uint32_t B = (uint32_t)(F - prev);
     const uint32_t h = B >> 1;
@@ -1239,7 +1248,7 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
       run = max(run, S.maxgap);
     }
     prev = absval(s, S.last);
-  }
+  });
   if (t == 1023) {
     A.final_out[0] = prev;
     A.final_out[1] = run;
@@ -1611,6 +1620,22 @@ hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st) {
   if (!n) return hipSuccess;
   const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(zero_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+__global__ void spans_kernel(SpanList L) {
+  const CopySpan S = L.s[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.words; i += (uint64_t)gridDim.x * blockDim.x)
+    S.dst[i] = S.src ? S.src[i] : 0u;
+}
+
+hipError_t launch_spans(const SpanList& L, hipStream_t st) {
+  if (L.n == 0) return hipSuccess;
+  if (L.n > (uint32_t)kMaxSpans) return hipErrorInvalidValue;
+  uint64_t mx = 1;
+  for (uint32_t j = 0; j < L.n; ++j) mx = std::max<uint64_t>(mx, L.s[j].words);
+  const uint64_t blocks = std::min<uint64_t>((mx + 255) / 256, 1024);
+  hipLaunchKernelGGL(spans_kernel, dim3((uint32_t)blocks, L.n), dim3(256), 0, st, L);
   return hipGetLastError();
 }
 
