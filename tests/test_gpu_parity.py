@@ -53,6 +53,22 @@ def test_full_stats_random_vs_oracle(voss, oracle):
         assert g["checkpoints"] == o["checkpoints"], (N, cp, thr)
 
 
+@pytest.mark.parametrize("N", [2_300_942_869, 2_300_942_900, 2_301_000_000])
+def test_record_in_the_last_segments_vs_oracle(voss, oracle, N):
+    # the maximal gap 320 ends at 2,300,942,869 (A002386/A005250), in the
+    # last, partial segment of the sweep: the records rescan and the
+    # checkpoint split run on a short final segment
+    g = voss.stats.full_stats(N, cp_interval=10**8, large_threshold=300)
+    o = oracle.full_stats(N, cp_interval=10**8, large_threshold=300)
+    for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"):
+        assert g[k] == o[k], k
+    assert g["hist"] == o["hist"]
+    assert g["records"] == o["records"]
+    assert g["large_gaps"] == o["large_gaps"]
+    assert g["checkpoints"] == o["checkpoints"]
+    assert g["records"][-1] == (2_300_942_869, 320)
+
+
 def test_ranges_near_the_top_vs_oracle(voss, oracle):
     # queries between 1e13 and the API's 1e14 ceiling: sieving primes up to
     # 1e7, most of them in the sparse tier of a short segment

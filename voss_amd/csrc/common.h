@@ -92,7 +92,7 @@ struct SieveArgs {
   uint32_t tier1_end;           // primes [tier0_end, tier1_end): one prime per wave, lanes split the segment
   uint32_t tier2_end;           // primes [tier1_end, tier2_end): one per lane, unrolled walk
   uint32_t tier3_end;           // primes [tier2_end, tier3_end): one per lane, <= 4 hits/class; rest (p >= seg_bytes): <= 1
-  unsigned int* work;           // batch mode: work counter (zeroed before launch)
+  unsigned int* work;           // segment work counter of the batch, count and stats modes (zeroed before launch)
   const uint32_t* nseg_dev;     // batch mode: if set, the segment count is read here
   // outputs (mode dependent; unused ones may be null)
   unsigned long long* totals;   // [0] count, [1] pi_4_1 part, [2] pi_3_1 part

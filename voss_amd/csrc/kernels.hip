@@ -118,7 +118,7 @@ uint32_t lds_bytes_for(uint32_t seg_bytes) { return (kLdsSeg * 4) + seg_bytes + 
 namespace {
 
 // misc word layout
-constexpr uint32_t kMiscItem = 0;       // current work item
+constexpr uint32_t kMiscItem = 0;       // next segment from the work counter
 constexpr uint32_t kMiscSieveCtr = 1;   // next crossing-off work items: tiers 0-3 (word 1), tier 4 (word 2)
 constexpr uint32_t kMiscSplit = 3;      // stats mode: checkpoint split bit, first and second bucket (words 3-5)
 constexpr uint32_t kMiscBounds = 160;   // segment tier bounds (6 words; above the per-wave records of 16 waves)
@@ -955,11 +955,17 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
   uint32_t cur_query = 0xFFFFFFFFu;
   uint32_t since_flush = 0;
   __syncthreads();
-  // Segments are dealt round-robin by workgroup index: the loop bound and the
-  // segment index are wave-uniform, and every thread-0 region below is
-  // followed by a barrier in uniform code (a divergent region at the loop
-  // latch lets the compiler hoist it into a divergent inner loop around the
-  // next barrier).
+  // Workgroup w starts on segment w.  Sweeps (count and stats) then take
+  // their next segment from the work counter (zeroed by the host): thread 64
+  // draws it while the current segment is sieved and every thread reads it
+  // at the loop latch, after the walk's barrier (the slot is rewritten only
+  // after the next init barrier, which wave 0 reaches after its merge), so
+  // workgroups that drew cheaper segments take more and all finish within
+  // one segment of each other (static round-robin: 1e11 sweep +1.1 %, 1e12
+  // +1 %).  Bitmap mode keeps round-robin.  The loop bound and the segment
+  // index are wave-uniform, and every thread-0 region below is followed by a
+  // barrier in uniform code (a divergent region at the loop latch lets the
+  // compiler hoist it into a divergent inner loop around the next barrier).
   const uint32_t nseg = (MODE == kModeBatch && A.nseg_dev) ? *A.nseg_dev : A.nseg;
   uint32_t s = blockIdx.x;
   if (MODE == kModeBatch) {
@@ -999,6 +1005,8 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       init_segment<BS>(A, lds, g);
     }
     __syncthreads();
+    if ((MODE == kModeStats || MODE == kModeCount) && threadIdx.x == 64)
+      lds[kLdsMisc + kMiscItem] = gridDim.x + atomicAdd(A.work, 1u);
     if (!cached) sieve_segment<BS>(A, lds, g);
     __syncthreads();
     if (MODE == kModeBitmap) {
@@ -1122,6 +1130,8 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       if (threadIdx.x == 0) lds[kLdsMisc + kMiscItem] = atomicAdd(A.work, 1u);
       __syncthreads();
       s = lds[kLdsMisc + kMiscItem];
+    } else if (MODE == kModeStats || MODE == kModeCount) {
+      s = __builtin_amdgcn_readfirstlane(lds[kLdsMisc + kMiscItem]);
     } else {
       s += gridDim.x;
     }
