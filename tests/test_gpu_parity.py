@@ -87,6 +87,19 @@ def test_ranges_near_the_top_vs_oracle(voss, oracle):
     assert voss.primes.in_range(10**14 - 10**5, 10**14) == oracle.primes_in_range(10**14 - 10**5, 10**14)
 
 
+def test_full_stats_numpy_output(voss):
+    # the numpy form (histogram buffer handed over uninitialised) equals the
+    # list form entry for entry, the zeros past the largest gap included
+    import numpy as np
+    g = voss.stats.full_stats(10**8, cp_interval=10**7)
+    for _ in range(2):
+        a = voss.stats.full_stats(10**8, cp_interval=10**7, as_numpy=True)
+        assert isinstance(a["hist"], np.ndarray) and len(a["hist"]) == len(g["hist"]) == 100000
+        assert a["hist"].tolist() == g["hist"]
+        assert [tuple(r) for r in a["checkpoints"].tolist()] == g["checkpoints"]
+    assert g["hist"][g["max_gap"] + 1:] == [0] * (100000 - g["max_gap"] - 1)
+
+
 def test_full_stats_large_gap_threshold(voss, oracle):
     # a low threshold makes the large-gap list long, exercising the append path
     g = voss.stats.full_stats(10**8, large_threshold=150)

@@ -148,7 +148,7 @@ static int ensure_computed(voss_primes_ctx* ctx, bool need_full) {
     ctx->count = S.prime_count;
     ctx->has_count = true;
     ctx->hist.assign(kMaxGap, 0);
-    for (int g = 0; g < kMaxGap; ++g) ctx->hist[g] = (int64_t)S.hist[g];
+    for (size_t g = 0; g < std::min<size_t>(kMaxGap, S.hist.size()); ++g) ctx->hist[g] = (int64_t)S.hist[g];
     ctx->c41 = ctx->c43 = 0;
     ctx->has_full = full;
     return VOSS_OK;
@@ -172,7 +172,7 @@ static int ensure_computed(voss_primes_ctx* ctx, bool need_full) {
     ctx->has_count = true;
     if (full) {
       ctx->hist.assign(kMaxGap, 0);
-      for (int g = 0; g < kMaxGap; ++g) ctx->hist[g] = (int64_t)S.hist[g];
+      for (size_t g = 0; g < std::min<size_t>(kMaxGap, S.hist.size()); ++g) ctx->hist[g] = (int64_t)S.hist[g];
       ctx->c41 = S.pi_4_1;
       ctx->c43 = S.pi_4_3;
       ctx->large = std::move(S.large);
@@ -548,7 +548,11 @@ extern "C" int voss_amd_full_stats(uint64_t N, uint64_t cp_interval, uint32_t la
     out->n_large = S.large_total;
     out->n_records = S.records.size();
     out->n_checkpoints = S.checkpoints.size() / 5;
-    if (hist) std::memcpy(hist, S.hist.data(), kMaxGap * sizeof(uint64_t));
+    if (hist) {
+      const size_t n = std::min<size_t>(kMaxGap, S.hist.size());
+      std::memcpy(hist, S.hist.data(), n * sizeof(uint64_t));
+      std::memset(hist + n, 0, (kMaxGap - n) * sizeof(uint64_t));
+    }
     if (checkpoints)
       std::memcpy(checkpoints, S.checkpoints.data(),
                   std::min<uint64_t>(cp_cap, S.checkpoints.size() / 5) * 5 * sizeof(uint64_t));

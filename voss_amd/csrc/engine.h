@@ -43,7 +43,7 @@ struct SweepStats {
   uint64_t pi_4_1 = 0, pi_4_3 = 0, pi_3_1 = 0, pi_3_2 = 0;
   uint64_t last_prime = 0;
   uint64_t max_gap = 0;
-  std::vector<uint64_t> hist;                // index = gap, size 100000
+  std::vector<uint64_t> hist;                // index = gap, up to the largest gap (all later counts are 0)
   std::vector<uint64_t> checkpoints;         // 5 per checkpoint: pi, c41, c43, c31, c32
   std::vector<GapEntry> large;               // gaps >= threshold, (gap desc, position asc)
   std::vector<GapEntry> records;             // maximal gaps, ascending position

@@ -372,7 +372,6 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
                                const ProgressFn& progress) {
   cfg_ = cfgs_[0];
   SweepStats R;
-  R.hist.assign(100000, 0);
   if (N < 2) throw Error(1, "N must be >= 2");
   const uint32_t S = cfg_.seg_bytes;
   if (cp_interval && cp_interval < 30ull * S)
@@ -564,8 +563,10 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   R.prime_count = nseed + cnt;
   // bins above the largest gap's (fin[1] bounds every gap) are zero; they
   // are downloaded only if that bound reaches past the staged ones
+  const bool all_bins = (fin[1] >> 1) >= kHistHead;
+  R.hist.assign(2 * (all_bins ? kGapBins : kHistHead), 0);
   for (uint32_t h = 1; h < kHistHead; ++h) R.hist[2 * h] += dh[h];
-  if ((fin[1] >> 1) >= kHistHead) {
+  if (all_bins) {
     std::vector<unsigned long long> rest(kGapBins - kHistHead);
     hip_check(hipMemcpy(rest.data(), d_hist.as<unsigned long long>() + kHistHead, rest.size() * 8,
                         hipMemcpyDeviceToHost), "hipMemcpy");
@@ -580,6 +581,7 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   R.last_prime = fin[0];
   for (uint64_t g2 = R.hist.size(); g2-- > 0;)
     if (R.hist[g2]) { R.max_gap = g2; break; }
+  R.hist.resize(R.max_gap + 1);
   if (ncp) {
     R.checkpoints.resize(5 * ncp);
     uint64_t c = 0, c41 = 0, c31 = 0;

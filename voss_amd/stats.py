@@ -28,7 +28,13 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
     if N < 2:
         raise VossInvalidNError(f"N must be >= 2, got {N}")
     st = _capi.AmdFullStats()
-    hist = (ctypes.c_uint64 * 100000)() if with_hist else None
+    if with_hist and as_numpy:
+        # uninitialised: the library writes all 100000 entries
+        import numpy as np
+        hist_np = np.empty(100000, dtype=np.uint64)
+        hist = hist_np.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    else:
+        hist = (ctypes.c_uint64 * 100000)() if with_hist else None
     ncp = N // cp_interval if cp_interval else 0
     cps = (ctypes.c_uint64 * max(1, 5 * ncp))()
     lg = (_capi.AmdGap * large_cap)()
@@ -42,7 +48,7 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
         "prime_count": st.prime_count, "pi_4_1": st.pi_4_1, "pi_4_3": st.pi_4_3,
         "pi_3_1": st.pi_3_1, "pi_3_2": st.pi_3_2, "last_prime": st.last_prime,
         "max_gap": st.max_gap, "n_large": st.n_large, "n_records": st.n_records,
-        "hist": (_np_view(hist) if as_numpy else list(hist)) if with_hist else None,
+        "hist": (hist_np if as_numpy else list(hist)) if with_hist else None,
         "checkpoints": (_np_view(cps)[:5 * ncp].reshape(ncp, 5) if as_numpy
                         else [tuple(cps[5 * j + f] for f in range(5)) for j in range(ncp)]),
         "large_gaps": [(lg[i].position, lg[i].gap) for i in range(nl)],
