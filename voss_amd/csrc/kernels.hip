@@ -1270,11 +1270,13 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
 
 // ---------------------------------------------------------------- records rescan
 // One workgroup per flagged wave range (the seg_bytes / waves bytes that one
-// wave of the sweep walked): re-sieve just that range, then wave 0 walks it
-// in order, 1 KiB per step, and lists every gap larger than all earlier gaps,
-// starting from the running maximum before the range and from the last prime
-// before it.  Each step takes the lanes' largest gaps, an exclusive prefix
-// maximum over the lanes, and lets the lanes whose chunk beats it emit.
+// wave of the sweep walked): re-sieve just that range and list every gap
+// larger than all earlier gaps, starting from the running maximum before the
+// range and from the last prime before it.  The range is walked in 1 KiB
+// steps, a wave per step: pass 1 takes each step's first and last prime and
+// largest inner gap, thread 0 chains the steps in order, and pass 2 re-walks
+// only the steps that hold a record, each lane emitting the gaps of its chunk
+// above the larger of the step's prior maximum and the earlier lanes' gaps.
 
 // Largest gap in a lane's 16-byte chunk at segment-relative byte boff (prev =
 // last prime before it, 0 if none), optionally emitting the gaps above `run`
@@ -1309,6 +1311,12 @@ __device__ __forceinline__ uint32_t chunk_gaps(uint4 q, uint32_t boff, uint32_t 
   return lm;
 }
 
+// 1 KiB steps of the largest wave range (seg_bytes / 8 waves of a 512-thread
+// block): five per-step arrays of this length fill misc words 16..115
+constexpr uint32_t kMaxRescanSteps = 20;
+static_assert(kMaxSegBytes / 8 <= kMaxRescanSteps * 1024, "rescan step records");
+static_assert(16 + 5 * kMaxRescanSteps <= kMiscBounds, "rescan step records below the segment bounds");
+
 template <int BS>
 __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   extern __shared__ uint32_t lds[];
@@ -1337,34 +1345,76 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
     sieve_segment<BS>(A, lds, g);
     __syncthreads();
 
-    if (wave == 0) {
-      const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
-      uint32_t run = item.threshold;
-      // values are segment-relative, as in the sweep
-      uint32_t carry = item.dprev ? 30 * off - item.dprev : 0;
-      for (uint32_t base = 0; base < g.nbytes; base += 1024) {
-        const uint32_t loc = base + 16 * lane;  // < sub: base is below nbytes <= sub
-        uint4 q = *reinterpret_cast<const uint4*>(seg8 + loc);
-        const uint32_t cb = 8 * loc;
-        if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
-        const uint32_t boff = off + loc;
-        uint32_t step_last;
-        const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
-        carry = max(carry, step_last);
-        const uint32_t lm = chunk_gaps<false>(q, boff, prev, 0, R, kseg, s);
-        // exclusive max of lm over the earlier lanes of this step
-        uint32_t incl = lm;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
-          if (lane >= (uint32_t)d) incl = max(incl, t);
-        }
-        uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
-        if (lane == 0) excl = 0;
-        const uint32_t r = max(run, excl);
-        if (lm > r) chunk_gaps<true>(q, boff, prev, r, R, kseg, s);
-        run = max(run, (uint32_t)__shfl((int)incl, 63, 64));
+    // Pass 1, a wave per 1 KiB step: the step's first and last prime and
+    // its largest gap between two of its own primes.  Values are
+    // segment-relative, as in the sweep.  Steps read below ceil(nbytes / 1 KiB)
+    // KiB <= sub bytes of the LDS segment; bits past nbytes are masked.
+    const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
+    const uint32_t nsteps = (g.nbytes + 1023) >> 10;  // <= kMaxRescanSteps
+    uint32_t* st_first = lds + kLdsMisc + 16;  // misc words 16..115: per-step records
+    uint32_t* st_last = st_first + kMaxRescanSteps;
+    uint32_t* st_max = st_last + kMaxRescanSteps;
+    uint32_t* st_thr = st_max + kMaxRescanSteps;    // largest gap before the step
+    uint32_t* st_carry = st_thr + kMaxRescanSteps;  // last prime before the step
+    auto load_chunk = [&](uint32_t st) {
+      const uint32_t loc = 1024 * st + 16 * lane;
+      uint4 q = *reinterpret_cast<const uint4*>(seg8 + loc);
+      const uint32_t cb = 8 * loc;
+      if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
+      return q;
+    };
+    for (uint32_t st = wave; st < nsteps; st += NW) {
+      const uint4 q = load_chunk(st);
+      const uint32_t boff = off + 1024 * st + 16 * lane;
+      uint32_t step_last;
+      const uint32_t prev = prev_in_step(chunk_last(q, boff), lane, step_last);
+      const uint32_t lm = wave_max(chunk_gaps<false>(q, boff, prev, 0, R, kseg, s));
+      const uint32_t cf = chunk_first(q, boff);
+      const uint64_t with = __ballot(cf != 0);
+      const uint32_t sf = with ? (uint32_t)__shfl((int)cf, (int)__builtin_ctzll(with), 64) : 0u;
+      if (lane == 0) {
+        st_first[st] = sf;
+        st_last[st] = step_last;
+        st_max[st] = lm;
       }
+    }
+    __syncthreads();
+    // the steps in order: the running maximum and the last prime before each
+    if (threadIdx.x == 0) {
+      uint32_t run = item.threshold;
+      uint32_t carry = item.dprev ? 30 * off - item.dprev : 0;
+      for (uint32_t st = 0; st < nsteps; ++st) {
+        st_thr[st] = run;
+        st_carry[st] = carry;
+        const uint32_t f = st_first[st];
+        if (!f) continue;
+        if (carry) run = max(run, f - carry);
+        run = max(run, st_max[st]);
+        carry = st_last[st];
+      }
+    }
+    __syncthreads();
+    // Pass 2, only for steps holding a gap above the maximum before them:
+    // each lane emits its chunk's gaps above the larger of that maximum and
+    // the earlier lanes' largest gap.
+    for (uint32_t st = wave; st < nsteps; st += NW) {
+      const uint32_t f = st_first[st], carry = st_carry[st], thr = st_thr[st];
+      if (!f || max(st_max[st], carry ? f - carry : 0u) <= thr) continue;  // wave-uniform
+      const uint4 q = load_chunk(st);
+      const uint32_t boff = off + 1024 * st + 16 * lane;
+      uint32_t step_last;
+      const uint32_t prev = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
+      const uint32_t lm = chunk_gaps<false>(q, boff, prev, 0, R, kseg, s);
+      uint32_t incl = lm;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)incl, d, 64);
+        if (lane >= (uint32_t)d) incl = max(incl, t);
+      }
+      uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
+      if (lane == 0) excl = 0;
+      const uint32_t r = max(thr, excl);
+      if (lm > r) chunk_gaps<true>(q, boff, prev, r, R, kseg, s);
     }
     __syncthreads();  // the next item rebuilds the range in LDS
   }
