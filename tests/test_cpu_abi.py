@@ -1,7 +1,8 @@
 """The C ABI without a GPU: libvoss.so loads, exports every function the
 headers in include/voss declare, the Python binding covers them all, argument
 checks fire before any device work, and device entry points fail loudly
-(VOSS_ERR_NO_CUDA) instead of falling back to the CPU."""
+(VOSS_ERR_CUDA, the reference's code for "no device") instead of falling back
+to the CPU."""
 
 import ctypes
 import re
@@ -95,18 +96,46 @@ def test_argument_checks_need_no_device(voss):
     assert (cfg.slots, cfg.max_batch, cfg.nth_max, cfg.goldbach_max) == (4, 1024, 10**12, 10**8)
 
 
-def test_device_paths_fail_loudly_without_a_device(voss):
-    if not _no_device(voss):
-        pytest.skip("a device is present")
-    from voss_amd import _capi
-    L = _capi._lib
-    out = ctypes.c_uint64()
-    assert L.voss_primes_prime_count(10**9, ctypes.byref(out)) == _capi.VOSS_ERR_NO_CUDA
-    assert L.voss_amd_server_start(None) == _capi.VOSS_ERR_NO_CUDA
-    with pytest.raises(voss.VossError):
-        voss.stats.range_stats(10, 100)
-    with pytest.raises(voss.VossError):
-        voss.primes.prime_count(10**6)
+_NO_DEVICE_SCRIPT = r"""
+import ctypes, sys
+sys.path.insert(0, sys.argv[1])
+import voss_amd as voss
+from voss_amd import _capi
+L = _capi._lib
+out = ctypes.c_uint64()
+E = _capi.VOSS_ERR_CUDA
+# the reference maps "no device" to VOSS_ERR_CUDA (prime_count.cu:147-151, 316-318)
+assert L.voss_primes_prime_count(10**9, ctypes.byref(out)) == E
+assert b"no HIP device" in L.voss_get_last_error()
+ctx = ctypes.c_void_p()
+assert L.voss_primes_ctx_new(10**6, 1, ctypes.byref(ctx)) == 0
+assert L.voss_primes_ctx_prime_count(ctx, ctypes.byref(out)) == E
+L.voss_primes_ctx_free(ctx)
+arr, n = ctypes.POINTER(ctypes.c_uint64)(), ctypes.c_uint64()
+assert L.voss_primes_in_range(1000, 2000, ctypes.byref(arr), ctypes.byref(n)) == E
+assert L.voss_primes_goldbach_test_upto(1000, ctypes.byref(arr), ctypes.byref(n), 10) == E
+assert L.voss_amd_server_start(None) == E
+# host-only entry points still answer
+assert L.voss_primes_prime_count(5, ctypes.byref(out)) == 0 and out.value == 3
+assert L.voss_primes_next(100, ctypes.byref(out)) == 0 and out.value == 101
+try:
+    voss.primes.prime_count(10**6)
+    raise SystemExit("no error raised")
+except voss.VossError:
+    pass
+print("ok")
+"""
+
+
+def test_device_paths_fail_loudly_without_a_device():
+    # a child process with the devices hidden, so this holds on a GPU box too
+    import os
+    import subprocess
+    import sys
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    r = subprocess.run([sys.executable, "-c", _NO_DEVICE_SCRIPT, str(ROOT)], env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
 
 
 def test_serving_trace_is_deterministic(voss):

@@ -31,12 +31,14 @@ def test_full_stats_1e11_vs_oracle(voss, oracle):
     assert g["records"][-1] == (42_652_618_343 + 464, 464)
 
 
-def test_context_progress_splits_1e11(voss):
-    # a progress callback splits the sweep into launches of about 5e10
-    # integers each (context.cu:260-269): at 1e11 that is two full launches
-    # and a last one of a few segments.  Every Context result must equal the
-    # single-launch sweep's and the published values.
+def test_context_progress_splits_1e11(voss, oracle):
+    # the progress callback fires once per reference segment of SEG_NUM =
+    # min(ceil(N/30)*30 + 10, 5e10 + 10) integers (context.cu:260-269,
+    # 433-436): at 1e11 two calls, (seg, 2, primes in seg, cumulative), the
+    # first segment being (0, 5e10 + 10].  Every Context result must equal
+    # the callback-free sweep's and the published values.
     N = 10**11
+    S = 50_000_000_010
 
     def results(c):
         # the result classes compare by identity, as the reference's do
@@ -50,11 +52,31 @@ def test_context_progress_splits_1e11(voss):
         split = results(c)
     with voss.primes.Context(N, profile="full") as c:
         whole = results(c)
-    assert [a[0] for a in seen] == [1, 2, 3] and all(a[1] == 3 for a in seen)
-    assert sum(a[2] for a in seen) == 4_118_054_813 - 3 and seen[-1][3] == 4_118_054_813
-    assert seen[0][3] > 1_900_000_000 and seen[1][3] == 4_118_054_813 - seen[2][2]
+    pi_s = oracle.prime_count(S)
+    assert seen == [(1, 2, pi_s - 3, pi_s), (2, 2, 4_118_054_813 - pi_s, 4_118_054_813)]
     assert split == whole
     assert split[0] == 4_118_054_813 and split[1] == 224_376_048
+
+
+@pytest.mark.parametrize("N", [5 * 10**10, 50_000_000_010, 50_000_000_011, 10**11 + 7, 150_000_000_031])
+def test_context_progress_segments(voss, oracle, N):
+    # one call below 5e10 + 10 (N = 5e10: SEG_NUM = 5e10 + 20, NUM_SEG = 1);
+    # N = 5e10 + 11 has a second segment of one integer, inside the kernel
+    # segment that the first launch already sieved
+    S = ((N + 29) // 30) * 30 + 10 if N < 50_000_000_010 else 50_000_000_010
+    nseg = (N + S - 1) // S
+    bounds = [min((j + 1) * S, N) for j in range(nseg)]
+    pis = [oracle.prime_count(b) for b in bounds]
+    want, prev = [], 3
+    for j, pi in enumerate(pis):
+        want.append((j + 1, nseg, pi - prev, pi))
+        prev = pi
+    for profile in ("minimal", "standard"):
+        seen = []
+        with voss.primes.Context(N, profile=profile) as c:
+            c.set_progress(lambda *a: seen.append(a))
+            assert c.prime_count() == pis[-1]
+        assert seen == want, (profile, seen, want)
 
 
 def _fixture():

@@ -25,7 +25,7 @@ namespace voss {
 
 namespace {
 const uint32_t kW30[8] = {1, 7, 11, 13, 17, 19, 23, 29};
-constexpr int kErrNoDevice = 3, kErrOom = 4, kErrDevice = 5, kErrArg = 6, kErrInternal = 7;
+constexpr int kErrOom = 4, kErrDevice = 5, kErrArg = 6, kErrInternal = 7;
 
 uint64_t isqrt64(uint64_t n) {
   uint64_t r = (uint64_t)std::sqrt((double)n);
@@ -88,8 +88,10 @@ Engine::Engine() {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n <= 0) {
+    // the reference throws here and its entry points map that to
+    // VOSS_ERR_CUDA (prime_count.cu:147-151, 316-318)
     (void)hipGetLastError();
-    throw Error(kErrNoDevice, "no HIP device available");
+    throw Error(kErrDevice, "no HIP device available");
   }
   hip_check(hipGetDevice(&dev_), "hipGetDevice");
   hipDeviceProp_t prop;
@@ -376,8 +378,7 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   const uint32_t S = cfg_.seg_bytes;
   if (cp_interval && cp_interval < 30ull * S)
     throw Error(kErrArg, "checkpoint interval must be at least " + std::to_string(30ull * S));
-  const uint64_t ncp = cp_interval ? N / cp_interval : 0;
-  const uint64_t nbuckets = cp_interval ? (N + cp_interval - 1) / cp_interval : 0;
+  const uint64_t ncp = cp_interval ? N / cp_interval : 0;  // checkpoints reported to the caller
   if (N < 7) return small_sweep_stats(N);
   const uint64_t nseed = 3;
   R.records.push_back({3, 1});
@@ -389,14 +390,30 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   const uint32_t nseg = (uint32_t)nseg64;
   const uint32_t large_cap = 1u << 20, rec_cap = 4096, rescan_cap = 4096, rout_cap = 16384;
 
-  // progress: one launch per reference-sized segment (5e10+10 integers,
-  // context.cu:260-269), otherwise a single launch
-  uint32_t per_launch = nseg;
+  // Progress reports per reference segment (context.cu:260-269, 433-436):
+  // SEG_NUM = min(ceil(N/30)*30 + 10, 5e10 + 10) integers, NUM_SEG =
+  // ceil(N / SEG_NUM) calls of (seg, NUM_SEG, primes in seg, cumulative).
+  // The kernel segments stay on their uniform grid: the checkpoint buckets,
+  // set to SEG_NUM, count each reference segment exactly (a kernel segment
+  // that straddles a boundary splits its tallies), and launch L ends with
+  // the kernel segment that holds reference segment L's last byte, so
+  // bucket L is final when launch L completes.
+  uint64_t ref_seg = 0, nref = 1;
   if (progress) {
-    const uint64_t ref_seg = 50000000010ull;
-    per_launch = (uint32_t)std::max<uint64_t>(1, ref_seg / (30ull * S));
+    if (cp_interval) throw Error(kErrArg, "a progress callback cannot be combined with checkpoints");
+    constexpr uint64_t kMaxRefSeg = 50000000010ull;
+    ref_seg = N < kMaxRefSeg ? ((N + 29) / 30) * 30 + 10 : kMaxRefSeg;
+    nref = (N + ref_seg - 1) / ref_seg;
+    if (nref > 1) cp_interval = ref_seg;
   }
-  const uint32_t nlaunch = (nseg + per_launch - 1) / per_launch;
+  const uint32_t nlaunch = (uint32_t)nref;
+  // kernel segments [launch_end[L-1], launch_end[L]) go to launch L
+  std::vector<uint32_t> launch_end(nlaunch, nseg);
+  for (uint32_t L = 0; L + 1 < nlaunch; ++L) {
+    const uint64_t bytes = (uint64_t)(L + 1) * (ref_seg / 30);  // reference segments 0..L
+    launch_end[L] = (uint32_t)std::min<uint64_t>(nseg, (bytes + S - 1) / S);
+  }
+  const uint64_t nbuckets = cp_interval ? (N + cp_interval - 1) / cp_interval : 0;
 
   const uint32_t waves = (uint32_t)cfg_.block / 64;
   DevBuf& d_sum = ws(kWsSummaries, (size_t)nseg * sizeof(SegSummary));
@@ -446,17 +463,21 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   uint64_t cumulative = nseed;
   for (uint32_t L = 0; L < nlaunch; ++L) {
     SieveArgs AL = A;
-    AL.seg_base = L * per_launch;
-    AL.nseg = std::min(per_launch, nseg - AL.seg_base);
+    AL.seg_base = L ? launch_end[L - 1] : 0;
+    AL.nseg = launch_end[L] - AL.seg_base;
     AL.work = d_work.as<unsigned int>() + L;
-    hip_check(launch_sieve(kModeStats, cfg_.block, AL, grid_for(AL.nseg), stream_), "launch(stats)");
+    // a reference segment wholly inside the previous launch's last kernel
+    // segment (N just past a boundary) needs no launch of its own
+    if (AL.nseg)
+      hip_check(launch_sieve(kModeStats, cfg_.block, AL, grid_for(AL.nseg), stream_), "launch(stats)");
     if (progress) {
+      // primes of reference segment L: its bucket, or every prime found so far
       unsigned long long c = 0;
-      hip_check(hipMemcpyAsync(&c, d_tot.p, 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy(progress)");
+      const void* src = nref > 1 ? (const void*)(d_bk.as<unsigned long long>() + 3 * L) : d_tot.p;
+      hip_check(hipMemcpyAsync(&c, src, 8, hipMemcpyDeviceToHost, stream_), "hipMemcpy(progress)");
       hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-      const uint64_t now = nseed + c;
-      progress((int)L + 1, (int)nlaunch, now - cumulative, now);
-      cumulative = now;
+      cumulative += c;
+      progress((int)L + 1, (int)nlaunch, c, cumulative);
     }
   }
   hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
