@@ -66,6 +66,18 @@ int voss_amd_range_stats(uint64_t a, uint64_t b, voss_amd_range_t* out);
 int voss_amd_batch_ranges(const uint64_t* a, const uint64_t* b, uint64_t n, voss_amd_range_t* out);
 /* Prime count in [a, b]. */
 int voss_amd_count_range(uint64_t a, uint64_t b, uint64_t* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the count of voss_amd_count_range, written to d_out[0]
+ * (device memory) by kernels enqueued on `stream` (a hipStream_t; NULL is the
+ * null stream).  It allocates nothing, copies nothing and does not wait: its
+ * work counter and per-segment counts live in a fixed 256 KiB device array
+ * per GPU, so [a, b] may span at most 16384 segments (7.2e10 integers at the
+ * default 144 KiB segments; VOSS_ERR_OUT_OF_RANGE beyond).  The library must
+ * already be initialised on the current device by an earlier device call
+ * (e.g. voss_amd_device_info); otherwise VOSS_ERR_INVALID_ARG.  Calls that
+ * overlap on one GPU race on the array: serialise them (one stream). */
+int voss_amd_count_range_noalloc(uint64_t a, uint64_t b, uint64_t* d_out, void* stream);
 
 /* GPU primality of n values (deterministic for every u64); out[i] = 0/1. */
 int voss_amd_is_prime_batch(const uint64_t* x, uint64_t n, uint8_t* out);

@@ -603,6 +603,21 @@ extern "C" int voss_amd_count_range(uint64_t a, uint64_t b, uint64_t* out) {
   });
 }
 
+extern "C" int voss_amd_count_range_noalloc(uint64_t a, uint64_t b, uint64_t* d_out, void* stream) {
+  if (!d_out) { set_err("d_out is null"); return VOSS_ERR_INVALID_ARG; }
+  if (b > kMaxN) { set_err("b exceeds maximum (10^14)"); return VOSS_ERR_OUT_OF_RANGE; }
+  return guarded([&] {
+    Engine* e = Engine::peek();  // never initialises (that allocates)
+    if (!e) {
+      set_err("the library is not initialised on this device: call a device entry point (voss_amd_device_info) first");
+      return VOSS_ERR_INVALID_ARG;
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->count_range_noalloc(a, b, reinterpret_cast<unsigned long long*>(d_out), static_cast<hipStream_t>(stream));
+    return VOSS_OK;
+  });
+}
+
 extern "C" int voss_amd_is_prime_batch(const uint64_t* x, uint64_t n, uint8_t* out) {
   if ((!x || !out) && n) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
   return guarded([&] {

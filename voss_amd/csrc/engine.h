@@ -104,9 +104,17 @@ struct NthIndex {
   std::vector<uint64_t> cum;
 };
 
+// The no-allocation count (Engine::count_range_noalloc) keeps its work
+// counter, partial count and per-segment counts in a fixed device array of
+// kNoAllocHeadBytes + kNoAllocMaxSegs * sizeof(SegSummary) bytes (about
+// 256 KiB), so it serves ranges of at most kNoAllocMaxSegs segments.
+constexpr uint32_t kNoAllocMaxSegs = 16384;
+constexpr size_t kNoAllocHeadBytes = 256;
+
 class Engine {
  public:
   static Engine& get();  // initialises the device on first use; throws if none
+  static Engine* peek();  // the engine if get() has succeeded before, else null
 
   // kind 0: sweeps (prime_count, full statistics, bitmaps); kind 1: batched ranges
   const TuneConfig& config(int kind = 0) const { return cfgs_[kind & 1]; }
@@ -143,6 +151,13 @@ class Engine {
   std::shared_ptr<const NthIndex> nth_index(uint64_t hi, const TuneConfig& c);
 
   KernelTimes last_times() const { return times_; }
+
+  // Prime count of [a, b] written to d_out[0] (device memory) by kernels
+  // enqueued on stream st, with no allocation, no copy and no wait: every
+  // buffer beyond d_out is the fixed scratch array.  Calls that overlap on
+  // the device race on that array.  Ranges of more than kNoAllocMaxSegs
+  // segments are rejected; the current device must be the engine's.
+  void count_range_noalloc(uint64_t a, uint64_t b, unsigned long long* d_out, hipStream_t st);
 
   // ---- batched ranges split into host layout, staging and read-back, so a
   // caller with its own stream (the request server) can run them
@@ -214,6 +229,10 @@ class Engine {
   uint32_t presieve_copy_words_ = 0;
   uint32_t tier2_div_ = 4;      // primes below seg_bytes / this: kernel tier 2
   KernelTimes times_;
+  // device address of the no-allocation scratch array (null if the lookup
+  // at start-up failed; scratch_err_ says why)
+  void* scratch_ = nullptr;
+  std::string scratch_err_;
 };
 
 void hip_check(hipError_t e, const char* what);

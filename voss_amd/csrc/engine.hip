@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <dlfcn.h>
 
 #include "kernels.h"
@@ -52,6 +53,13 @@ float elapsed(const Event& a, const Event& b) {
 }
 }  // namespace
 
+// Scratch of Engine::count_range_noalloc, the one entry point that never
+// allocates: bytes [0, 4) the segment work counter, [8, 16) the kernel's
+// running total, then one SegSummary per segment (kNoAllocMaxSegs at most).
+// Each GPU holds its own copy; host code uses only the address that
+// hipGetSymbolAddress returns for it (Engine::scratch_), never this name.
+static __device__ __align__(256) uint8_t scratch[kNoAllocHeadBytes + (size_t)kNoAllocMaxSegs * sizeof(SegSummary)];
+
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error(kErrDevice, std::string(what) + " failed: " + hipGetErrorString(e));
 }
@@ -79,10 +87,17 @@ DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
   return *this;
 }
 
+namespace {
+std::atomic<Engine*> g_engine{nullptr};
+}
+
 Engine& Engine::get() {
   static Engine* inst = new Engine();  // lives until process exit
+  g_engine.store(inst, std::memory_order_release);
   return *inst;
 }
+
+Engine* Engine::peek() { return g_engine.load(std::memory_order_acquire); }
 
 Engine::Engine() {
   int n = 0;
@@ -158,6 +173,19 @@ Engine::Engine() {
   // sieving primes up to 2^24 cover every value below 2.8e14; built once so
   // concurrent users (the request server's slots) never see them move
   ensure_tables(1ull << 48);
+  // the device address of this GPU's scratch array (the engine serves one
+  // device, current here); a failed lookup disables only the no-allocation
+  // entry point
+  {
+    void* p = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (se == hipSuccess && p) {
+      scratch_ = p;
+    } else {
+      (void)hipGetLastError();
+      scratch_err_ = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(se);
+    }
+  }
 }
 
 Engine::~Engine() {}
@@ -337,6 +365,43 @@ uint64_t Engine::count_bitmap(uint64_t lo, uint64_t hi) {
   times_.total_ms = times_.sieve_ms;
   times_.segments = nseg;
   return c;
+}
+
+void Engine::count_range_noalloc(uint64_t a, uint64_t b, unsigned long long* d_out, hipStream_t st) {
+  if (!scratch_) throw Error(kErrInternal, scratch_err_);
+  int cur = -1;
+  hip_check(hipGetDevice(&cur), "hipGetDevice");
+  if (cur != dev_) throw Error(kErrArg, "the current device is not the one the library was initialised on");
+  const TuneConfig c = cfgs_[0];
+  const uint32_t S = c.seg_bytes;
+  uint64_t seeds = 0;
+  for (uint64_t sd : {2ull, 3ull, 5ull}) seeds += (sd >= a && sd <= b) ? 1 : 0;
+  const uint64_t lo = std::max<uint64_t>(a, 7);
+  uint8_t* base = static_cast<uint8_t*>(scratch_);
+  SegSummary* sums = reinterpret_cast<SegSummary*>(base + kNoAllocHeadBytes);
+  uint32_t nseg = 0;
+  if (b >= lo) {
+    const uint64_t k_base = lo / 30, k_end = b / 30 + 1;
+    const uint64_t nseg64 = (k_end - k_base + S - 1) / S;
+    if (nseg64 > kNoAllocMaxSegs)
+      throw Error(2, "range needs " + std::to_string(nseg64) + " segments; the no-allocation count holds at most " +
+                         std::to_string(kNoAllocMaxSegs) + " (" + std::to_string(30ull * S * kNoAllocMaxSegs) +
+                         " integers)");
+    nseg = (uint32_t)nseg64;
+    // the work counter and running total start at zero (a fill kernel: a
+    // memset node would not replay correctly inside a captured graph)
+    hip_check(launch_zero(reinterpret_cast<uint32_t*>(base), kNoAllocHeadBytes / 4, st), "launch(zero)");
+    SieveArgs A = base_args(nseg, c);
+    A.k_base = k_base;
+    A.k_end = k_end;
+    A.lo_val = lo;
+    A.hi_val = b;
+    A.work = reinterpret_cast<unsigned int*>(base);
+    A.totals = reinterpret_cast<unsigned long long*>(base + 8);
+    A.summaries = sums;  // every segment's count is written: nothing to clear
+    hip_check(launch_sieve(kModeCount, c.block, A, grid_for(nseg, c), st), "launch(count)");
+  }
+  hip_check(launch_sum_counts(sums, nseg, seeds, d_out, st), "launch(sum_counts)");
 }
 
 uint64_t Engine::count_range(uint64_t lo, uint64_t hi) {

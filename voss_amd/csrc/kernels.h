@@ -66,6 +66,9 @@ hipError_t launch_select(int block, const SieveArgs& A, const uint32_t* segs, co
                          unsigned long long* out, const uint32_t* n_dev, uint32_t n_cap, hipStream_t st);
 // Zero n 32-bit words (graph-safe fill, no memset node).
 hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st);
+// out[0] = add + sums[0].count + ... + sums[n-1].count (one workgroup).
+hipError_t launch_sum_counts(const SegSummary* sums, uint32_t n, uint64_t add, unsigned long long* out,
+                             hipStream_t st);
 
 // Up to kMaxSpans word ranges zeroed (src null) or copied, in one launch: the
 // sweep clears its counters and packs its results for one download this way

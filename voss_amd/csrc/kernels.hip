@@ -1683,6 +1683,28 @@ hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// out[0] = add + the counts of segments [0, n), summed by one workgroup.
+__global__ void __launch_bounds__(1024) sum_counts_kernel(const SegSummary* sums, uint32_t n, uint64_t add,
+                                                           unsigned long long* out) {
+  __shared__ unsigned long long part[16];
+  unsigned long long c = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) c += sums[i].count;
+  c = wave_sum(c);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = add;
+    for (int w = 0; w < 16; ++w) t += part[w];
+    out[0] = t;
+  }
+}
+
+hipError_t launch_sum_counts(const SegSummary* sums, uint32_t n, uint64_t add, unsigned long long* out,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(sum_counts_kernel, dim3(1), dim3(1024), 0, st, sums, n, add, out);
+  return hipGetLastError();
+}
+
 __global__ void spans_kernel(SpanList L) {
   const CopySpan S = L.s[blockIdx.y];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.words; i += (uint64_t)gridDim.x * blockDim.x)
