@@ -90,6 +90,16 @@ int voss_amd_prev_prime_batch(const uint64_t* x, uint64_t n, uint64_t* out);
 int voss_amd_nth_prime(uint64_t n, uint64_t* out);
 /* Goldbach partition counts (p1 <= p2) for even n >= 4, one launch for all. */
 int voss_amd_goldbach_counts(const uint64_t* ns, uint64_t k, uint64_t* out);
+/* voss_primes_goldbach_test_upto with two more knobs: partners restricted to
+ * primes p <= pmax (0: no restriction; a restriction plants counterexamples:
+ * the even n in [6, N] with no p <= min(n/2, pmax) such that n - p is prime),
+ * and the first pass's partner list cut at scan_limit (0: every sieving
+ * prime), past which its candidates are settled by exact partition counts.
+ * Writes the first max_ce counterexamples in ascending order (malloc'd, free
+ * with voss_free) and, if out_total is not NULL, how many there are in all.
+ * N <= 1e12. */
+int voss_amd_goldbach_test_upto_pmax(uint64_t N, uint64_t pmax, uint64_t scan_limit, uint64_t max_ce,
+                                     uint64_t** out_ce, uint64_t* out_count, uint64_t* out_total);
 
 /* ---------------------------------------------------------------- serving
  * A continuous-batching request server (one per process).  Requests of mixed

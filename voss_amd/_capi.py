@@ -113,6 +113,7 @@ _SIGNATURES = {
     "voss_primes_goldbach_count": ([_u64, _pu64], _int),
     "voss_primes_goldbach_partitions": ([_u64, _ppu64, _pu64], _int),
     "voss_primes_goldbach_test_upto": ([_u64, _ppu64, _pu64, _u64], _int),
+    "voss_amd_goldbach_test_upto_pmax": ([_u64, _u64, _u64, _u64, _ppu64, _pu64, _pu64], _int),
     "voss_primes_ctx_new": ([_u64, _int, _P(_vp)], _int),
     "voss_primes_ctx_free": ([_vp], None),
     "voss_primes_ctx_prime_count": ([_vp, _pu64], _int),

@@ -476,6 +476,9 @@ extern "C" int voss_primes_goldbach_partitions(uint64_t n, uint64_t** out_array,
   }, VOSS_ERR_INTERNAL);
 }
 
+// The wheel bitmap of [0, N] (N/30 bytes) must fit the device: N <= 1e12.
+constexpr uint64_t kGoldbachMaxN = 1000000000000ull;
+
 extern "C" int voss_primes_goldbach_test_upto(uint64_t N, uint64_t** out_ce, uint64_t* out_count,
                                               uint64_t max_ce) {  // goldbach_upto.cu:48-150
   if (!out_ce || !out_count) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
@@ -483,6 +486,7 @@ extern "C" int voss_primes_goldbach_test_upto(uint64_t N, uint64_t** out_ce, uin
   *out_count = 0;
   if (N < 4) { set_err("N must be >= 4"); return VOSS_ERR_INVALID_N; }
   if (max_ce == 0) return VOSS_OK;
+  if (N > kGoldbachMaxN) { set_err("N exceeds 10^12 (the prime bitmap of [0, N] must fit the device)"); return VOSS_ERR_OUT_OF_RANGE; }
   return guarded([&] {
     std::vector<uint64_t> ce;
     {
@@ -490,6 +494,29 @@ extern "C" int voss_primes_goldbach_test_upto(uint64_t N, uint64_t** out_ce, uin
       std::lock_guard<std::mutex> lk(e.mu);
       ce = e.goldbach_upto(N, max_ce);
     }
+    if (ce.empty()) return VOSS_OK;
+    *out_ce = to_malloc(ce);
+    *out_count = ce.size();
+    return VOSS_OK;
+  }, VOSS_ERR_INTERNAL);
+}
+
+extern "C" int voss_amd_goldbach_test_upto_pmax(uint64_t N, uint64_t pmax, uint64_t scan_limit, uint64_t max_ce,
+                                                uint64_t** out_ce, uint64_t* out_count, uint64_t* out_total) {
+  if (!out_ce || !out_count) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
+  *out_ce = nullptr;
+  *out_count = 0;
+  if (N < 4) { set_err("N must be >= 4"); return VOSS_ERR_INVALID_N; }
+  if (N > kGoldbachMaxN) { set_err("N exceeds 10^12 (the prime bitmap of [0, N] must fit the device)"); return VOSS_ERR_OUT_OF_RANGE; }
+  return guarded([&] {
+    std::vector<uint64_t> ce;
+    uint64_t total = 0;
+    {
+      Engine& e = eng();
+      std::lock_guard<std::mutex> lk(e.mu);
+      ce = e.goldbach_upto(N, max_ce, pmax, scan_limit, out_total ? &total : nullptr);
+    }
+    if (out_total) *out_total = total;
     if (ce.empty()) return VOSS_OK;
     *out_ce = to_malloc(ce);
     *out_count = ce.size();

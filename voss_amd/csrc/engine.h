@@ -140,8 +140,15 @@ class Engine {
   uint64_t bitmap_range(uint64_t lo, uint64_t hi, DevBuf& out, uint64_t* k_base);
 
   // Callers around the sieve (engine_callers.hip)
-  std::vector<uint64_t> goldbach_upto(uint64_t N, uint64_t max_ce);       // counterexamples
+  // Goldbach counterexamples in [4, N], ascending, the first max_ce; partners
+  // limited to p <= pmax (0: none), first-pass scan to scan_limit (0: the
+  // table); *total (if given) receives how many there are in all
+  std::vector<uint64_t> goldbach_upto(uint64_t N, uint64_t max_ce, uint64_t pmax = 0, uint64_t scan_limit = 0,
+                                      uint64_t* total = nullptr);
   std::vector<uint64_t> goldbach_counts(const uint64_t* ns, size_t k);    // partition counts
+  // partition counts (p1 <= plim) over a bitmap of [0, >= nmax] built by bitmap_range
+  std::vector<uint64_t> goldbach_counts_bm(const DevBuf& bm, const uint64_t* ns, size_t k, uint64_t nmax,
+                                           uint64_t plim);
   uint64_t sophie_germain(uint64_t N);                                    // p <= N, 2p+1 prime
   void is_prime_batch(const uint64_t* x, size_t n, uint8_t* out);
   void next_prev_batch(const uint64_t* x, size_t n, uint64_t* out, int dir);

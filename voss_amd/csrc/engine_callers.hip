@@ -21,43 +21,96 @@ namespace {
 constexpr int kErrArg = 6, kErrInternal = 7;
 }
 
-std::vector<uint64_t> Engine::goldbach_upto(uint64_t N, uint64_t max_ce) {
+// Even n in [6, N] with no partition p1 + p2 = n, p1 <= p2, p1 <= pmax (0:
+// no limit; a limit plants counterexamples to test with), ascending, the
+// first max_ce of them (every one counted into *total when given; n = 4 =
+// 2 + 2 always has one).  A first pass (goldbach_check_kernel) tries the
+// odd primes up to min(N/2, pmax, scan_limit if set, the sieving table's
+// 2^24); an n left without a partner is final when that list held every
+// partner <= min(n/2, pmax), and is otherwise settled by an exact count of
+// its partitions over those partners.  Candidates arrive in windows of at
+// most kCap even n when the whole range yields more than kCap of them.
+std::vector<uint64_t> Engine::goldbach_upto(uint64_t N, uint64_t max_ce, uint64_t pmax, uint64_t scan_limit,
+                                            uint64_t* total) {
   std::vector<uint64_t> ce;
-  if (N < 4 || max_ce == 0) return ce;
+  if (total) *total = 0;
+  if (N < 6 || (max_ce == 0 && !total)) return ce;
+  const uint64_t n_top = N & ~1ull;
+  const uint64_t plim = pmax ? pmax : ~0ull;
   DevBuf bm;
   uint64_t k_base = 0;
-  bitmap_range(7, std::max<uint64_t>(N, 7), bm, &k_base);  // k_base = 0
-  // ascending odd primes 3, 5, ..., up to min(N/2, table limit).  An n with no
-  // partner among them is only a candidate; candidates are settled below by
-  // an exact partition count over every prime <= n/2.
+  bitmap_range(7, N, bm, &k_base);  // k_base = 0
+  uint64_t lim = std::min<uint64_t>(n_top / 2, plim);
+  if (scan_limit) lim = std::min(lim, scan_limit);
+  lim = std::min<uint64_t>(lim, table_limit_);
   std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19};  // the table starts at 23
-  while (!sp.empty() && (uint64_t)sp.back() > N / 2) sp.pop_back();
+  while (!sp.empty() && (uint64_t)sp.back() > lim) sp.pop_back();
   for (uint32_t p : h_primes_) {
-    if ((uint64_t)p > N / 2) break;
+    if ((uint64_t)p > lim) break;
     sp.push_back(p);
   }
-  DevBuf d_sp(sp.size() * 4 + 4), d_ce(std::min<uint64_t>(max_ce, 1u << 20) * 8 + 8), d_n(8);
-  hip_check(hipMemcpyAsync(d_sp.p, sp.data(), sp.size() * 4, hipMemcpyHostToDevice, stream_), "H2D");
-  hip_check(hipMemsetAsync(d_n.p, 0, 8, stream_), "memset");
-  const uint64_t cap = std::min<uint64_t>(max_ce, 1u << 20);
-  hip_check(launch_goldbach_check(bm.as<uint8_t>(), d_sp.as<uint32_t>(), (uint32_t)sp.size(), N,
-                                  d_ce.as<uint64_t>(), d_n.as<unsigned long long>(), cap, stream_),
-            "launch(goldbach_check)");
-  unsigned long long n = 0;
-  hip_check(hipMemcpyAsync(&n, d_n.p, 8, hipMemcpyDeviceToHost, stream_), "D2H");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  if (n > cap) throw Error(kErrInternal, "too many Goldbach candidates");
-  std::vector<uint64_t> cand(n);
-  if (n) {
+  constexpr uint64_t kCap = 1u << 20;
+  DevBuf d_sp(sp.size() * 4 + 4), d_ce(kCap * 8), d_n(8);
+  if (!sp.empty())
+    hip_check(hipMemcpyAsync(d_sp.p, sp.data(), sp.size() * 4, hipMemcpyHostToDevice, stream_), "H2D");
+  auto pass = [&](uint64_t lo, uint64_t hi) -> uint64_t {
+    hip_check(hipMemsetAsync(d_n.p, 0, 8, stream_), "memset");
+    hip_check(launch_goldbach_check(bm.as<uint8_t>(), d_sp.as<uint32_t>(), (uint32_t)sp.size(), lo, hi,
+                                    d_ce.as<uint64_t>(), d_n.as<unsigned long long>(), kCap, stream_),
+              "launch(goldbach_check)");
+    unsigned long long n = 0;
+    hip_check(hipMemcpyAsync(&n, d_n.p, 8, hipMemcpyDeviceToHost, stream_), "D2H");
+    hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    return n;
+  };
+  // take the candidates of one pass in order; true when the caller needs no more
+  auto take = [&](uint64_t n) -> bool {
+    if (!n) return false;
+    std::vector<uint64_t> cand(n);
     hip_check(hipMemcpy(cand.data(), d_ce.p, n * 8, hipMemcpyDeviceToHost), "D2H");
     std::sort(cand.begin(), cand.end());
-    const bool exhaustive = table_limit_ >= N / 2;  // every prime <= N/2 was tried
-    std::vector<uint64_t> cnt = exhaustive ? std::vector<uint64_t>(cand.size(), 0)
-                                           : goldbach_counts(cand.data(), cand.size());
-    for (size_t i = 0; i < cand.size() && ce.size() < max_ce; ++i)
-      if (cnt[i] == 0) ce.push_back(cand[i]);
+    // n is final when the scan held every partner <= min(n/2, pmax)
+    std::vector<uint64_t> open;
+    for (uint64_t c : cand)
+      if (std::min(c / 2, plim) > lim) open.push_back(c);
+    std::vector<uint64_t> cnt;
+    if (!open.empty()) cnt = goldbach_counts_bm(bm, open.data(), open.size(), N, plim);
+    size_t j = 0;
+    for (uint64_t c : cand) {
+      bool none = true;
+      if (j < open.size() && open[j] == c) none = cnt[j++] == 0;
+      if (!none) continue;
+      if (total) ++*total;
+      if (ce.size() < max_ce) ce.push_back(c);
+      else if (!total) return true;
+    }
+    return !total && ce.size() >= max_ce;
+  };
+  const uint64_t all = pass(6, n_top);
+  if (all <= kCap) {
+    take(all);
+    return ce;
+  }
+  for (uint64_t lo = 6; lo <= n_top; lo += 2 * kCap) {
+    const uint64_t hi = std::min<uint64_t>(n_top, lo + 2 * (kCap - 1));
+    if (take(pass(lo, hi))) break;
   }
   return ce;
+}
+
+std::vector<uint64_t> Engine::goldbach_counts_bm(const DevBuf& bm, const uint64_t* ns, size_t k, uint64_t nmax,
+                                                 uint64_t plim) {
+  std::vector<uint64_t> out(k, 0);
+  if (!k) return out;
+  DevBuf d_ns(k * 8), d_out(k * 8);
+  hip_check(hipMemcpyAsync(d_ns.p, ns, k * 8, hipMemcpyHostToDevice, stream_), "H2D");
+  for (size_t i0 = 0; i0 < k; i0 += 65535)
+    hip_check(launch_goldbach_count(bm.as<uint8_t>(), d_ns.as<uint64_t>() + i0, std::min<size_t>(65535, k - i0), nmax,
+                                    plim, d_out.as<unsigned long long>() + i0, stream_),
+              "launch(goldbach_count)");
+  hip_check(hipMemcpyAsync(out.data(), d_out.p, k * 8, hipMemcpyDeviceToHost, stream_), "D2H");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  return out;
 }
 
 std::vector<uint64_t> Engine::goldbach_counts(const uint64_t* ns, size_t k) {
@@ -71,15 +124,7 @@ std::vector<uint64_t> Engine::goldbach_counts(const uint64_t* ns, size_t k) {
   DevBuf bm;
   uint64_t k_base = 0;
   bitmap_range(7, nmax, bm, &k_base);
-  DevBuf d_ns(k * 8), d_out(k * 8);
-  hip_check(hipMemcpyAsync(d_ns.p, ns, k * 8, hipMemcpyHostToDevice, stream_), "H2D");
-  for (size_t i0 = 0; i0 < k; i0 += 65535)
-    hip_check(launch_goldbach_count(bm.as<uint8_t>(), d_ns.as<uint64_t>() + i0, std::min<size_t>(65535, k - i0), nmax,
-                                    d_out.as<unsigned long long>() + i0, stream_),
-              "launch(goldbach_count)");
-  hip_check(hipMemcpyAsync(out.data(), d_out.p, k * 8, hipMemcpyDeviceToHost, stream_), "D2H");
-  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  return out;
+  return goldbach_counts_bm(bm, ns, k, nmax, ~0ull);
 }
 
 uint64_t Engine::sophie_germain(uint64_t N) {

@@ -10,6 +10,7 @@
 //   core/miller_rabin.cpp:26-51     deterministic MR, bases 2..37
 //   primes/next_prev.cu:22-83       odd scan with MR
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <stdint.h>
 
 #include "misc_kernels.h"
@@ -213,22 +214,28 @@ __global__ void next_prev_kernel(const uint64_t* x, uint64_t n, uint64_t* out, i
   out[i] = res;
 }
 
-// One thread per even n = 4 + 2*idx: search the ascending small primes for a
-// partner n - p in the bitmap (goldbach_upto.cu:25-46).
-__global__ void goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t N,
-                                      uint64_t* ce, unsigned long long* nce, uint64_t max_ce) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t n = 4 + 2 * idx;
-  if (n > N) return;
-  if (n == 4) return;  // 2 + 2
-  const uint64_t half = n / 2;
-  for (uint32_t i = 0; i < nsp; ++i) {
-    const uint64_t p = sp[i];
-    if (p > half) break;
-    if (bm_prime(bm, n - p)) return;
+// Even n in [n_lo, n_hi] (both even, n_lo >= 6), a grid-stride loop over
+// them: search the ascending partners sp[] (odd primes, those <= n/2 only)
+// for one with n - p in the bitmap (goldbach_upto.cu:25-46) and append the n
+// that have none.  The slot counter keeps counting past cap, so the caller
+// sees how many there were.
+__global__ void goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t n_lo,
+                                      uint64_t n_hi, uint64_t* ce, unsigned long long* nce, uint64_t cap) {
+  const uint64_t count = (n_hi - n_lo) / 2 + 1;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < count;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t n = n_lo + 2 * idx;
+    const uint64_t half = n / 2;
+    bool found = false;
+    for (uint32_t i = 0; i < nsp; ++i) {
+      const uint64_t p = sp[i];
+      if (p > half) break;
+      if (bm_prime(bm, n - p)) { found = true; break; }
+    }
+    if (found) continue;
+    const unsigned long long slot = atomicAdd(nce, 1ull);
+    if (slot < cap) ce[slot] = n;
   }
-  const unsigned long long slot = atomicAdd(nce, 1ull);
-  if (slot < max_ce) ce[slot] = n;
 }
 
 // Goldbach partition counts as bit-plane correlations of the wheel bitmap.
@@ -244,17 +251,18 @@ constexpr uint32_t kGbThreads = 256, kGbIters = 16;
 constexpr uint64_t kGbBytesPerBlock = 4ull * kGbThreads * kGbIters;
 
 __global__ void __launch_bounds__(kGbThreads) goldbach_count_kernel(const uint8_t* bm, const uint64_t* ns,
-                                                                    unsigned long long* out) {
+                                                                    uint64_t plim, unsigned long long* out) {
   __shared__ unsigned long long red[kGbThreads / 64];
   const uint64_t n = ns[blockIdx.y];
-  const uint64_t half = n / 2;
+  // partitions p1 + p2 = n with p1 <= p2 and p1 <= plim
+  const uint64_t half = min(n / 2, plim);
   const uint64_t kmax = half / 30;  // last low byte that can hold p <= n/2
   const uint64_t kbeg = (uint64_t)blockIdx.x * kGbBytesPerBlock;
   if (n < 4 || kbeg > kmax) return;  // uniform per workgroup
   unsigned long long c = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // p in {2, 3, 5}: 2 + 2 = 4; 3 and 5 with any prime partner q >= p
-    if (n == 4) c = 1;
+    if (n == 4 && half >= 2) c = 1;
     if (n > 4) {
       if (3 <= half && bm_prime(bm, n - 3)) ++c;
       if (5 <= half && bm_prime(bm, n - 5)) ++c;
@@ -348,22 +356,23 @@ hipError_t launch_next_prev(const uint64_t* x, uint64_t n, uint64_t* out, int di
   hipLaunchKernelGGL(next_prev_kernel, dim3((uint32_t)((n + 63) / 64)), dim3(64), 0, st, x, n, out, dir);
   return hipGetLastError();
 }
-hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t N, uint64_t* ce,
-                                 unsigned long long* nce, uint64_t max_ce, hipStream_t st) {
-  if (N < 4) return hipSuccess;
-  const uint64_t cnt = (N - 4) / 2 + 1;
-  hipLaunchKernelGGL(goldbach_check_kernel, dim3((uint32_t)((cnt + 255) / 256)), dim3(256), 0, st, bm, sp, nsp, N,
-                     ce, nce, max_ce);
+hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t n_lo, uint64_t n_hi,
+                                 uint64_t* ce, unsigned long long* nce, uint64_t cap, hipStream_t st) {
+  if (n_lo < 6 || n_hi < n_lo || (n_lo & 1) || (n_hi & 1)) return hipErrorInvalidValue;
+  const uint64_t cnt = (n_hi - n_lo) / 2 + 1;
+  const uint64_t blocks = std::min<uint64_t>((cnt + 255) / 256, 1u << 20);  // the kernel strides over the rest
+  hipLaunchKernelGGL(goldbach_check_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, bm, sp, nsp, n_lo, n_hi, ce,
+                     nce, cap);
   return hipGetLastError();
 }
-hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, uint64_t n_max,
+hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, uint64_t n_max, uint64_t plim,
                                  unsigned long long* out, hipStream_t st) {
   if (!k) return hipSuccess;
   if (k > 65535) return hipErrorInvalidValue;  // grid.y limit; callers split larger sets
   hipLaunchKernelGGL(zero_u64_kernel, dim3((uint32_t)((k + 255) / 256)), dim3(256), 0, st, out, k);
   const uint64_t chunks = (n_max / 2 / 30) / kGbBytesPerBlock + 1;
   hipLaunchKernelGGL(goldbach_count_kernel, dim3((uint32_t)chunks, (uint32_t)k), dim3(kGbThreads), 0, st, bm, ns,
-                     out);
+                     plim, out);
   return hipGetLastError();
 }
 hipError_t launch_sophie(const uint8_t* bm, uint64_t N, unsigned long long* out, hipStream_t st) {

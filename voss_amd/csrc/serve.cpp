@@ -298,7 +298,7 @@ void Server::launch(Slot& s) {
       for (uint32_t i : s.by_kind[kReqGoldbach]) gmax = std::max(gmax, s.items[i].req.a);
       hip_check(launch_goldbach_count(gb_bitmap_.as<uint8_t>(),
                                       reinterpret_cast<const uint64_t*>(din + s.in_off[kReqGoldbach]), nG, gmax,
-                                      reinterpret_cast<unsigned long long*>(dout + s.out_off[kReqGoldbach]), s.st),
+                                      ~0ull, reinterpret_cast<unsigned long long*>(dout + s.out_off[kReqGoldbach]), s.st),
                 "launch(goldbach)");
     }
     if (nT) {

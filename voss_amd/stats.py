@@ -168,6 +168,18 @@ def goldbach_counts(ns: list) -> list:
 KIND = {"both": -1, "sweep": 0, "batch": 1}
 
 
+def goldbach_test_upto(N: int, max_ce: int = 10, pmax: int = 0, scan_limit: int = 0):
+    """(total, first max_ce counterexamples ascending) among the even n <= N
+    when Goldbach partners are restricted to primes <= pmax (0: none); the
+    first pass tries partners up to scan_limit (0: all sieving primes) and
+    settles the rest by exact counts."""
+    arr = ctypes.POINTER(ctypes.c_uint64)()
+    cnt, tot = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _capi._check(_L.voss_amd_goldbach_test_upto_pmax(N, pmax, scan_limit, max_ce, ctypes.byref(arr),
+                                                     ctypes.byref(cnt), ctypes.byref(tot)), _ERRMAP)
+    return tot.value, _capi.take_u64(arr, cnt.value)
+
+
 def set_config(seg_bytes: int, block: int = 1024, wgs_per_cu: int = 1, tier1_p: int = 2048,
                kind: str = "both") -> None:
     """Engine launch configuration for sweeps, batched ranges, or both."""
