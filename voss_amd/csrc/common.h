@@ -10,11 +10,16 @@
 
 namespace voss {
 
-// Primes 7..19 are removed by AND-ing a periodic byte pattern (period
-// 7*11*13*17*19 = 323,323 bytes) into LDS; sieving proper starts at 23.
-constexpr int kPresieveGroups = 1;
-constexpr uint32_t kPresievePeriods[kPresieveGroups] = {7u * 11u * 13u * 17u * 19u};
-constexpr uint32_t kFirstSievingPrime = 23;
+// Primes 7..19 are removed by copying a periodic byte pattern (period
+// 7*11*13*17*19 = 323,323 bytes) into LDS as the segment's initial bytes;
+// 23..59 by AND-ing three more patterns (periods 23*29*31 = 20,677,
+// 37*41*43 = 65,231 and 47*53*59 = 147,029 bytes; the "read patterns") into
+// every 16-byte chunk where the sieved segment is read: crossing-off never
+// reads the bitmap, so the AND may come last.  Crossing-off starts at 61.
+constexpr int kPresieveGroups = 4;
+constexpr uint32_t kPresievePeriods[kPresieveGroups] = {7u * 11u * 13u * 17u * 19u, 23u * 29u * 31u, 37u * 41u * 43u,
+                                                        47u * 53u * 59u};
+constexpr uint32_t kFirstSievingPrime = 61;
 // Primes below this are sieved by tier 0 (a half-wave per prime and class).
 constexpr uint32_t kTier0P = 192;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
@@ -79,7 +84,7 @@ struct SieveArgs {
   uint32_t pi_tab_len;
   const uint32_t* presieve;     // pattern bytes as u32 words, kPresieveGroups patterns
   uint32_t presieve_off[kPresieveGroups];  // word offset of each pattern
-  uint32_t presieve_copy_words;  // pattern 0 is stored 4 times, copy c shifted by c bytes, this many words apart
+  uint32_t presieve_copy_words[kPresieveGroups];  // each pattern is stored 4 times, copy c shifted by c bytes, this many words apart
   uint32_t first_word;          // true prime bitmap of bytes 0..3 (values < 120)
   // geometry
   uint64_t k_base;              // sweep: byte index of segment 0

@@ -224,7 +224,7 @@ class Engine {
   TuneConfig cfgs_[2];
   TuneConfig cfg_;  // the configuration of the call in flight
   // resident tables
-  std::vector<uint32_t> h_primes_;  // sieving primes >= 23 (plain values)
+  std::vector<uint32_t> h_primes_;  // sieving primes >= kFirstSievingPrime (plain values)
   uint32_t* d_primes_ = nullptr;
   uint32_t* d_pitab_ = nullptr;
   uint32_t pitab_len_ = 0;
@@ -233,7 +233,7 @@ class Engine {
   uint32_t first_word_ = 0;
   uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
   uint32_t tier0_p_ = kTier0P;  // primes below: kernel tier 0
-  uint32_t presieve_copy_words_ = 0;
+  uint32_t presieve_copy_words_[kPresieveGroups] = {0};
   uint32_t tier2_div_ = 4;      // primes below seg_bytes / this: kernel tier 2
   KernelTimes times_;
   // device address of the no-allocation scratch array (null if the lookup

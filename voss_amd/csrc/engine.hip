@@ -16,8 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
-#include <cstring>
 #include <atomic>
+#include <cstring>
 #include <dlfcn.h>
 
 #include "kernels.h"
@@ -132,8 +132,10 @@ Engine::Engine() {
 
   // presieve patterns: in pattern g, byte j has bit o set iff 30j + W30[o] has
   // no factor among the group's primes; each is stored for one period plus a
-  // full segment so a segment never wraps
-  static const uint32_t groups[kPresieveGroups][5] = {{7, 11, 13, 17, 19}};
+  // full segment so a segment never wraps.  Pattern 0 is a segment's initial
+  // bytes, the others (the read patterns) are ANDed in as it is read.
+  static const uint32_t groups[kPresieveGroups][5] = {
+      {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0}, {37, 41, 43, 0, 0}, {47, 53, 59, 0, 0}};
   std::vector<uint32_t> words;
   for (int gi = 0; gi < kPresieveGroups; ++gi) {
     presieve_off_[gi] = (uint32_t)words.size();
@@ -151,7 +153,7 @@ Engine::Engine() {
     }
     // four copies, copy c starting c bytes in, so a read at any byte phase
     // starts on a word of one of them (the kernel copies it to LDS directly)
-    presieve_copy_words_ = (uint32_t)(len / 4);
+    presieve_copy_words_[gi] = (uint32_t)(len / 4);
     for (int c = 0; c < 4; ++c) {
       const size_t w0 = words.size();
       words.resize(w0 + len / 4);
@@ -291,7 +293,7 @@ SieveArgs Engine::base_args(uint32_t nseg, const TuneConfig& c) const {
   A.pi_tab_len = pitab_len_;
   A.presieve = d_presieve_;
   for (int k = 0; k < kPresieveGroups; ++k) A.presieve_off[k] = presieve_off_[k];
-  A.presieve_copy_words = presieve_copy_words_;
+  for (int k = 0; k < kPresieveGroups; ++k) A.presieve_copy_words[k] = presieve_copy_words_[k];
   A.first_word = first_word_;
   A.seg_bytes = c.seg_bytes;
   A.nseg = nseg;

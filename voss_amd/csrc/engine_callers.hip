@@ -43,7 +43,7 @@ std::vector<uint64_t> Engine::goldbach_upto(uint64_t N, uint64_t max_ce, uint64_
   uint64_t lim = std::min<uint64_t>(n_top / 2, plim);
   if (scan_limit) lim = std::min(lim, scan_limit);
   lim = std::min<uint64_t>(lim, table_limit_);
-  std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19};  // the table starts at 23
+  std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};  // the table starts at 61
   while (!sp.empty() && (uint64_t)sp.back() > lim) sp.pop_back();
   for (uint32_t p : h_primes_) {
     if ((uint64_t)p > lim) break;

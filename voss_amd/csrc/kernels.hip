@@ -212,9 +212,7 @@ __device__ __forceinline__ SegBounds segment_bounds(const SieveArgs& A, const Se
 template <int BS>
 __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   uint8_t* seg8 = reinterpret_cast<uint8_t*>(lds + kLdsSeg);
-  uint32_t off[kPresieveGroups];
-#pragma unroll
-  for (int k = 0; k < kPresieveGroups; ++k) off[k] = (uint32_t)(g.k0 % kPresievePeriods[k]);
+  const uint32_t off0 = (uint32_t)(g.k0 % kPresievePeriods[0]);
   const uint32_t nchunks = ((g.nbytes + 1023) & ~1023u) >> 4;
   if (threadIdx.x < 2) lds[kLdsMisc + kMiscSieveCtr + threadIdx.x] = 0;  // sieve_segment's work counters
   // thread 0 computes the tier bounds first, so their table loads overlap its
@@ -226,9 +224,8 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   // once), from the copy whose byte phase makes the source word-aligned.
   // Threads fill the rest: block 0 of segment 0 (its true bits patched in),
   // the partial last block and the zero fill up to the next 1 KiB.
-  static_assert(kPresieveGroups == 1, "one presieve pattern");
-  const uint32_t ph = off[0] & 3;
-  const uint32_t* src = A.presieve + A.presieve_off[0] + ph * A.presieve_copy_words + ((off[0] - ph) >> 2);
+  const uint32_t ph = off0 & 3;
+  const uint32_t* src = A.presieve + A.presieve_off[0] + ph * A.presieve_copy_words[0] + ((off0 - ph) >> 2);
   const uint32_t nfull = g.nbytes >> 8;
   const uint32_t first = (g.k0 < 4 && nfull > 0) ? 1u : 0u;  // else the thread pass covers chunk 0
   {
@@ -307,13 +304,52 @@ __device__ void cache_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   }
 }
 
+// The read patterns (multiples of 23..59 cleared) over one segment: the 16
+// bytes that AND into the segment's chunk at byte boff (a multiple of 16),
+// each from the copy whose phase makes the source word-aligned.  In a
+// segment starting below global byte 4, chunk 0 keeps its first 4 - k0
+// bytes as they are: init_segment wrote the true primes there (23..59 among
+// them).
+constexpr int kReadPatterns = kPresieveGroups - 1;
+struct ReadPatterns {
+  const uint32_t* src[kReadPatterns];  // word of pattern 1 + i that aligns with segment byte 0
+  uint32_t head_keep;                  // chunk 0, word 0: bits left untouched
+};
+__device__ __forceinline__ ReadPatterns read_patterns(const SieveArgs& A, uint64_t k0) {
+  ReadPatterns P;
+#pragma unroll
+  for (int i = 0; i < kReadPatterns; ++i) {
+    const uint32_t off = (uint32_t)(k0 % kPresievePeriods[1 + i]);
+    const uint32_t ph = off & 3;
+    P.src[i] = A.presieve + A.presieve_off[1 + i] + ph * A.presieve_copy_words[1 + i] + ((off - ph) >> 2);
+  }
+  P.head_keep = k0 < 4 ? (0xFFFFFFFFu >> (8 * (uint32_t)k0)) : 0u;
+  return P;
+}
+__device__ __forceinline__ uint4 read_pattern_chunk(const ReadPatterns& P, uint32_t boff) {
+  uint4 v = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+  for (int i = 0; i < kReadPatterns; ++i) {
+    const uint32_t* p = P.src[i] + (boff >> 2);
+    v.x &= p[0]; v.y &= p[1]; v.z &= p[2]; v.w &= p[3];
+  }
+  if (boff == 0) v.x |= P.head_keep;
+  return v;
+}
+__device__ __forceinline__ void and4(uint4& q, const uint4& b) {
+  q.x &= b.x; q.y &= b.y; q.z &= b.z; q.w &= b.w;
+}
+
 __device__ __forceinline__ void lds_and_at(uint32_t addr, uint32_t m) {
   __hip_atomic_fetch_and((lds_u32*)(uintptr_t)addr, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // LDS byte address of the segment bitmap.
-__device__ __forceinline__ uint32_t seg_base_addr(uint32_t* lds) {
-  return (uint32_t)(uintptr_t)(lds_u32*)lds + kLdsSeg * 4;
+__device__ __forceinline__ uint32_t seg_base_addr(uint32_t*) {
+  // dynamic LDS starts at address 0 (prepare_kernels checks that no static
+  // LDS precedes it), so the base is a constant the ds instructions take in
+  // their offset field
+  return kLdsSeg * 4;
 }
 
 // x mod p for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
@@ -794,6 +830,13 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
   const uint32_t wbeg = wave * wave_bytes;
   const uint32_t wend = min(wbeg + wave_bytes, g.nbytes);
+  // the read patterns are ANDed in as each chunk is read (not for batch
+  // segments filled from the resident cache, whose bytes are final); the
+  // next step's are loaded one step ahead
+  const bool apply_rp = !(MODE == kModeBatch && g.from_cache);  // uniform per workgroup
+  const ReadPatterns RP = read_patterns(A, g.k0);
+  uint4 bnext = make_uint4(~0u, ~0u, ~0u, ~0u);
+  if (apply_rp && wbeg < wend) bnext = read_pattern_chunk(RP, wbeg + 16 * lane);
   // waves further along drop in issue priority, so lagging waves catch up
   // and the walk's waves reach the barrier together
   __builtin_amdgcn_s_setprio(3);
@@ -806,6 +849,11 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     }
     const uint32_t boff = base + 16 * lane;
     uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
+    if (apply_rp) {
+      const uint4 b = bnext;
+      if (base + 1024 < wend) bnext = read_pattern_chunk(RP, boff + 1024);
+      and4(q, b);
+    }
     if (partial) {
       const uint32_t cb = 8 * boff;
       if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
@@ -1013,8 +1061,10 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
       uint8_t* dst = A.bitmap + (g.k0 - A.k_base);
       const uint32_t n16 = (g.nbytes + 15) >> 4;
+      const ReadPatterns RP = read_patterns(A, g.k0);
       for (uint32_t t = threadIdx.x; t < n16; t += BS) {
         uint4 q = *reinterpret_cast<const uint4*>(seg8 + 16 * t);
+        and4(q, read_pattern_chunk(RP, 16 * t));
         const uint32_t cb = 128 * t;
         if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
         *reinterpret_cast<uint4*>(dst + 16 * t) = q;
@@ -1356,9 +1406,11 @@ __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
     uint32_t* st_max = st_last + kMaxRescanSteps;
     uint32_t* st_thr = st_max + kMaxRescanSteps;    // largest gap before the step
     uint32_t* st_carry = st_thr + kMaxRescanSteps;  // last prime before the step
+    const ReadPatterns RP = read_patterns(A, g.k0);
     auto load_chunk = [&](uint32_t st) {
       const uint32_t loc = 1024 * st + 16 * lane;
       uint4 q = *reinterpret_cast<const uint4*>(seg8 + loc);
+      and4(q, read_pattern_chunk(RP, loc));
       const uint32_t cb = 8 * loc;
       if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
       return q;
@@ -1453,9 +1505,11 @@ __global__ void __launch_bounds__(BS) select_kernel(SieveArgs A, const uint32_t*
   const uint32_t wave_bytes = A.seg_bytes / NW;
   const uint32_t wbeg = wave * wave_bytes, wend = min(wbeg + wave_bytes, g.nbytes);
   uint32_t* wcount = lds + kLdsMisc + 16;  // NW words
+  const ReadPatterns RP = read_patterns(A, g.k0);
   uint32_t c = 0;
   for (uint32_t base = wbeg; base < wend; base += 1024) {
     uint4 q = *reinterpret_cast<const uint4*>(seg8 + base + 16 * lane);
+    and4(q, read_pattern_chunk(RP, base + 16 * lane));
     const uint32_t cb = 8 * (base + 16 * lane);
     if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
     c += popc4(q);
@@ -1478,6 +1532,7 @@ __global__ void __launch_bounds__(BS) select_kernel(SieveArgs A, const uint32_t*
   uint32_t need = rank - before;  // 1-based within this wave's region
   for (uint32_t base = wbeg; base < wend; base += 1024) {
     uint4 q = *reinterpret_cast<const uint4*>(seg8 + base + 16 * lane);
+    and4(q, read_pattern_chunk(RP, base + 16 * lane));
     const uint32_t boff = base + 16 * lane;
     const uint32_t cb = 8 * boff;
     if (cb < g.vlo || cb + 128 > g.vhi) mask_chunk(q, cb, g.vlo, g.vhi);
