@@ -441,6 +441,22 @@ __device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t 
   lds_and_at(a & ~3u, ~(1u << (((a & 3) << 3) | bit)));
 }
 
+// The crossings of a prime p >= nbytes (at most one per class), every class
+// checked in turn with no search and no loop: off = (cc_c - rk) mod p.
+__device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, uint32_t r, uint32_t rk,
+                                                   uint32_t nbytes, const uint8_t* tab) {
+  const uint32_t q30 = p / 30;
+  const uint32_t prk = p - rk;
+  const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
+  const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
+    const uint32_t off = min(cc + prk, cc - rk);
+    if (off < nbytes) cross_one(base, off, (uint32_t)((bits >> (8 * c)) & 7));
+  }
+}
+
 // Every crossing of p (rk = k0 mod p) in the segment, class by class, from
 // two packed 64-bit LDS rows: off = min(cc + p - rk, cc - rk), the p*p rule
 // applied where it can bite.  LONG walks each class with the unrolled
@@ -484,8 +500,8 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
 //   tier 3 (seg_bytes/4 <= p < seg_bytes): one prime per lane, at most four
 //     crossings per class, so the per-class setup is kept to a few
 //     instructions (cross_prime_classes);
-//   tier 4 (p >= seg_bytes): one prime per lane, only the classes whose single
-//     crossing falls inside the segment are visited.
+//   tier 4 (p >= seg_bytes): one prime per lane, all eight classes checked
+//     for their single possible crossing (cross_classes_once).
 template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
@@ -574,13 +590,14 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     it = nit;
     e = ne;
   }
-  // tier 4 (p >= seg_bytes): at most one crossing per class.  The classes'
-  // first multiples cc_c (byte offsets within one period p) increase with c,
-  // so the classes that hit the window [rk, rk + nbytes) mod p form one run
-  // (two when the window wraps past p): a three-step search finds its start
-  // and only the run is visited.  Items are groups of g4 64-prime chunks
-  // (about eight per wave: fine enough to balance the waves, few enough to
-  // bound the counter traffic of a short segment with thousands of chunks)
+  // tier 4 (p >= seg_bytes): at most one crossing per class, each of the
+  // eight classes checked in turn, fully unrolled and predicated (a search
+  // for the run of classes that hit, with a loop over it, cost more: its
+  // divergent loops and their exec-mask upkeep outweighed the classes it
+  // skipped, even for primes 20 segments long).  Items are groups of g4
+  // 64-prime chunks (about eight per wave: fine enough to balance the waves,
+  // few enough to bound the counter traffic of a short segment with
+  // thousands of chunks)
   // taken from a second LDS counter; the primes of the next chunk, and of
   // the next item's first chunk, are loaded ahead.
   const uint32_t c4 = (np - n3 + 63) >> 6;
@@ -616,35 +633,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
         cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
         continue;
       }
-      const uint32_t q30 = p / 30;
-      const uint2 fr = *reinterpret_cast<const uint2*>(tab + r * 8);
-      const uint2 br = *reinterpret_cast<const uint2*>(tab + 64 + r * 8);
-      auto cc = [&](uint32_t c) { return __umul24(q30, w30(c)) + __builtin_amdgcn_perm(fr.y, fr.x, c | 0x0C0C0C00u); };
-      auto bit_of = [&](uint32_t c) { return __builtin_amdgcn_perm(br.y, br.x, c | 0x0C0C0C00u) & 7u; };
-      // c = number of classes with cc_c < rk.  cc_c = q30 W[c] + frac_c with
-      // frac_c < 30 <= q30, so with w = floor(rk / q30) every class with W < w
-      // counts, every class with W > w does not, and only W == w needs a look.
-      uint32_t w = (uint32_t)((float)rk * __builtin_amdgcn_rcpf((float)q30));
-      if ((uint32_t)__umul24(w, q30) > rk) --w;
-      if ((uint32_t)__umul24(w + 1, q30) <= rk) ++w;
-      const uint64_t below = w < 16 ? 0x4433222211111100ull : 0x8877777766665544ull;  // #W30 < w, 4 bits each
-      uint32_t c = (uint32_t)(below >> (4 * (w & 15))) & 15u;
-      if (c < 8 && cc(c) < rk) ++c;
-      const uint32_t end = rk + nbytes;
-      for (; c < 8; ++c) {
-        const uint32_t x = cc(c);
-        if (x >= end) break;
-        cross_one(base, x - rk, bit_of(c));
-      }
-      if (end > p) {
-        const uint32_t end2 = end - p;
-        for (uint32_t c2 = 0; c2 < 8; ++c2) {
-          const uint32_t x = cc(c2);
-          if (x >= end2) break;
-          cross_one(base, x + p - rk, bit_of(c2));
-        }
-      }
-
+      cross_classes_once(base, p, r, rk, nbytes, tab);
     }
     it4 = nit4;
     f4 = nf4;
