@@ -469,6 +469,7 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
   const bool low = p >= plow;  // the p*p rule can bite
+  const uint32_t rot = (p & 3) << 3;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
@@ -481,11 +482,15 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
     const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
     if (LONG) {
       if (off < nbytes) cross_class(base, off, p, nbytes, bit);
-    } else {
-      while (off < nbytes) {
-        cross_one(base, off, bit);
+    } else if (off < nbytes) {
+      // one crossing per step: off advances by p, so the byte within the
+      // dword advances by p mod 4 and the mask rotates by 8 (p mod 4) bits
+      uint32_t m = ~(1u << (((off & 3) << 3) | bit));
+      do {
+        lds_and_at(base + (off & ~3u), m);
         off += p;
-      }
+        m = __builtin_rotateleft32(m, rot);
+      } while (off < nbytes);
     }
   }
 }
