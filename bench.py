@@ -74,6 +74,43 @@ def crossings(N: int, primes_upto=small_primes_upto) -> int:
     return total
 
 
+def range_crossings(a: int, b: int, primes) -> int:
+    """Crossings the wheel-30 sieve of [max(a, 7), b] must make: for each prime
+    7 <= p <= sqrt(b), the m coprime to 30 with p*m in [max(a, 7, p*p), b]
+    (primes: ascending numpy int64 array of the primes >= 7 up to sqrt(b))."""
+    import numpy as np
+    lo = max(a, 7)
+    P = primes[: np.searchsorted(primes, math.isqrt(b), side="right")]
+    if not len(P):
+        return 0
+    below = np.array([sum(1 for r in (1, 7, 11, 13, 17, 19, 23, 29) if r <= x) for x in range(30)], dtype=np.int64)
+
+    def cop30(x):
+        return 8 * (x // 30) + below[x % 30]
+    m_lo = np.maximum((lo + P - 1) // P, P)
+    m_hi = b // P
+    ok = m_hi >= m_lo
+    return int((cop30(m_hi[ok]) - cop30(m_lo[ok] - 1)).sum())
+
+
+def host_info():
+    """CPU model and the compiler the oracle is built with (same image here and on the GPU box)."""
+    import subprocess
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        cc = "unknown"
+    return {"cpu_model": model, "compiler": cc + ", -O3 -march=x86-64-v3 (oracle/Makefile), pthreads"}
+
+
 def cpu_baseline(sample_n: int, reps: int):
     from oracle import oracle as O
     O.build()
@@ -84,9 +121,42 @@ def cpu_baseline(sample_n: int, reps: int):
         s = O.full_stats(sample_n, cp_interval=10**9, threads=threads)
         times.append(time.perf_counter() - t0)
     dt = sorted(times)[len(times) // 2]
-    return {"value": sample_n / dt, "unit": "ints/s", "cores": threads, "kind": "port",
+    return {"value": sample_n / dt, "unit": "ints/s", "cores": threads, "kind": "port", **host_info(),
             "sample": f"full statistics over [2, {sample_n:.0e}] (same outputs as the GPU step), "
                       f"median of {reps}: {dt:.2f} s wall, pi = {s['prime_count']}"}
+
+
+def cpu_other_configs(st, gold):
+    """The oracle on the host cores beside the GPU for the other configurations:
+    C0 pi(1e9) whole; the 4096-range batch on its first 256 queries; the
+    [2, 1e12] sweep on its top 2e10 integers (range statistics there: the same
+    sieve work per integer as the sweep's hardest stretch)."""
+    from oracle import oracle as O
+    threads = os.cpu_count() or 1
+    out = {}
+    t0 = time.perf_counter()
+    pi = O.prime_count(10**9, threads)
+    cpu_s = time.perf_counter() - t0
+    import voss_amd
+    voss_amd.primes.prime_count(10**9)  # warm
+    t0 = time.perf_counter()
+    g = voss_amd.primes.prime_count(10**9)
+    gpu_s = time.perf_counter() - t0
+    out["c0_prime_count_1e9"] = {"cpu_s": cpu_s, "gpu_s": gpu_s, "gpu_kernel_ms": st.last_kernel_ms()["sieve_ms"],
+                                 "pi": pi, "exact": pi == g == 50_847_534, "cores": threads}
+    qs = gold[:256]
+    t0 = time.perf_counter()
+    ok = all(O.range_stats(q[0], q[1], threads)["count"] == q[2] for q in qs)
+    dt = time.perf_counter() - t0
+    out["c2_batch_sample"] = {"queries": len(qs), "cpu_s": dt, "cpu_qps": len(qs) / dt, "exact": ok, "cores": threads,
+                              "sample": "the first 256 of the 4096 ranges, one oracle range_stats call each"}
+    a, b = 10**12 - 2 * 10**10, 10**12
+    t0 = time.perf_counter()
+    O.range_stats(a, b, threads)
+    dt = time.perf_counter() - t0
+    out["c4_sweep_sample"] = {"cpu_ints_per_s": (b - a) / dt, "cpu_s": dt, "cores": threads,
+                              "sample": f"range statistics over [{a}, {b}] (the sweep's top 2%)"}
+    return out
 
 
 FIXTURE_RANGES_4096 = ROOT / "tests" / "golden" / "ranges_4096.json"
@@ -111,6 +181,9 @@ def bench_batch(st, reps: int):
     a = [g[0] for g in gold]
     b = [g[1] for g in gold]
     st.batch_ranges(a, b, as_numpy=True)  # warm: plan capture
+    import numpy as np
+    P = np.array([p for p in small_primes_upto(math.isqrt(max(b))) if p >= 7], dtype=np.int64)
+    algo_bytes = sum(4 * (range_crossings(x, y, P) + g[2]) + 2 * ((y - max(x, 7)) // 30 + 1) for x, y, g in zip(a, b, gold))
     times, kms = [], []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -124,9 +197,15 @@ def bench_batch(st, reps: int):
     plain = [st.range_stats(x, y) for x, y in zip(a, b)]
     dt_plain = time.perf_counter() - t0
     exact_plain = all(p["count"] == g[2] and p["first"] == g[3] and p["last"] == g[4] for p, g in zip(plain, gold))
+    dev_ms = sorted(kms)[len(kms) // 2]
     return {"workload": "4096 ranges, seed 0, a=10^U(6,13), width=10^U(3,9); count, gap histogram (bins 2..512), "
                         "first/last",
-            "batched_qps": len(a) / dt, "batched_ms": dt * 1e3, "batched_device_ms": sorted(kms)[len(kms) // 2],
+            "batched_qps": len(a) / dt, "batched_ms": dt * 1e3, "batched_device_ms": dev_ms,
+            "roofline": {"bound": "lds", "achieved": algo_bytes / (dev_ms / 1e3) / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
+                         "unit": "GB/s", "frac": algo_bytes / (dev_ms / 1e3) / LDS_PEAK_BYTES,
+                         "algorithmic_lds_bytes": algo_bytes,
+                         "note": "4 B per crossing and per prime, 2 B per 30 integers (init + count), over the "
+                                 "graph's device time (sieve kernel plus its small copies and stitch)"},
             "plain_qps": len(a) / dt_plain, "plain_ms": dt_plain * 1e3,
             "speedup": dt_plain / dt, "exact_vs_fixture": bool(exact and exact_plain),
             "engine": st.get_config("batch")}
@@ -148,7 +227,8 @@ def bench_serve(n: int):
         res[mode] = r
         out[mode] = {"qps": st["qps"], "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
                      "max_ms": st["max_ms"], "ticks": st["ticks"], "max_tick": st["max_tick"],
-                     "saturated_qps": sat["qps"], "warmup_s": st["warmup_s"],
+                     "saturated_qps": sat["qps"], "saturated_p50_ms": sat["p50_ms"],
+                     "saturated_p99_ms": sat["p99_ms"], "warmup_s": st["warmup_s"],
                      "errors": int((r["status"] != 0).sum())}
     a, b = res["served"], res["plain"]
     out["identical"] = bool(np.array_equal(a["status"], b["status"]) and np.array_equal(a["value"], b["value"])
@@ -174,7 +254,9 @@ def bench_large(st):
     r = st.full_stats(N, cp_interval=f["checkpoint_interval"], as_numpy=True)
     st.synchronize()
     sweep_s = time.perf_counter() - t0
-    sweep_dev_ms = st.last_kernel_ms()["total_ms"]
+    km = st.last_kernel_ms()
+    sweep_dev_ms = km["total_ms"]
+    algo = 4 * (crossings(N) + r["prime_count"]) + 2 * (N // 30 + 1)
     hist = {str(i): int(c) for i, c in enumerate(r["hist"][:1501]) if c}
     exact = (all(r[k] == f[k] for k in ("prime_count", "pi_4_1", "pi_4_3", "pi_3_1", "pi_3_2", "last_prime", "max_gap"))
              and hist == f["hist_1_1500"] and [list(x) for x in r["records"]] == f["records"]
@@ -189,6 +271,10 @@ def bench_large(st):
     return {"workload": "full statistics over [2, 1e12] (tallies every 1e10); Goldbach: every even n in [4, 1e9] "
                         "has a partition, partition counts at n = j*1e6, j = 1..1000",
             "sweep_s": sweep_s, "sweep_ints_per_s": N / sweep_s, "sweep_device_ms": sweep_dev_ms,
+            "sweep_sieve_ms": km["sieve_ms"],
+            "roofline": {"bound": "lds", "achieved": algo / (km["sieve_ms"] / 1e3) / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
+                         "unit": "GB/s", "frac": algo / (km["sieve_ms"] / 1e3) / LDS_PEAK_BYTES,
+                         "algorithmic_lds_bytes": algo},
             "goldbach_check_s": gb_s, "goldbach_counterexamples": g["counterexamples"],
             "goldbach_counts_s": cnt_s,
             "exact_vs_fixture": bool(exact and counts == f["goldbach_counts"] and not g["counterexamples"])}
@@ -244,7 +330,9 @@ def main():
     C = crossings(N)
     sieve_ms = sum(k["sieve_ms"] for k in kernel_ms) / len(kernel_ms)
     algo_ops = C + r["prime_count"]
-    achieved = 4 * algo_ops / (sieve_ms / 1e3)  # bytes/s: one LDS dword per algorithmic op
+    init_count_bytes = 2 * (N // 30 + 1)  # each bitmap byte written once (init) and read once (walk)
+    algo_bytes = 4 * algo_ops + init_count_bytes
+    achieved = algo_bytes / (sieve_ms / 1e3)  # bytes/s
     traffic = None
     tf = ROOT / "profiles" / "traffic_full_stats_1e11.json"
     if tf.exists():
@@ -276,8 +364,8 @@ def main():
         "roofline": {"bound": "lds", "achieved": achieved / 1e9, "peak": LDS_PEAK_BYTES / 1e9,
                      "unit": "GB/s", "frac": achieved / LDS_PEAK_BYTES,
                      "traffic": traffic,
-                     "algorithmic_ops_per_launch": algo_ops, "algorithmic_lds_bytes_per_launch": 4 * algo_ops,
-                     "crossings": C,
+                     "algorithmic_ops_per_launch": algo_ops, "algorithmic_lds_bytes_per_launch": algo_bytes,
+                     "init_and_count_bytes": init_count_bytes, "crossings": C,
                      "note": "LDS-resident sieve: HBM carries only tables and per-segment summaries"},
     }
     if not args.no_batch:
@@ -288,6 +376,7 @@ def main():
         line["large"] = bench_large(st)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(int(args.cpu_sample), args.cpu_reps)
+        line["other_configs"] = cpu_other_configs(st, json.loads(FIXTURE_RANGES_4096.read_text())["queries"])
     print(json.dumps(line))
 
 
