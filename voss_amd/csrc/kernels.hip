@@ -160,7 +160,7 @@ __device__ void init_tables(uint32_t* lds) {
     uint32_t res = prod % 30, bit = 0;
     for (int o = 0; o < 8; ++o) if (w30_const(o) == res) bit = o;
     tab[tid] = (uint8_t)(prod / 30);                                     // FRAC
-    tab[64 + tid] = (uint8_t)bit;                                        // BITS
+    tab[64 + tid] = (uint8_t)~(1u << bit);                               // NOTBIT: the class bit cleared
     tab[128 + tid] = (uint8_t)((w30_const(c) + 30 - w30_const(r)) % 30); // DM
   }
 }
@@ -352,6 +352,23 @@ __device__ __forceinline__ uint32_t seg_base_addr(uint32_t*) {
   return kLdsSeg * 4;
 }
 
+// Crossing masks.  mc = 0xFFFFFF00 | nb, nb the byte ~(1 << bit) of a class
+// (table row NOTBIT).  The mask that clears the class bit of the byte at
+// segment offset off is mc rotated left by 8 (off mod 4) bits: alignbit
+// rotates right by its shift's low five bits, and off * (2^24 - 8) has low
+// five bits -8 off mod 32 (off < 2^24).  A prime's next crossing is p bytes
+// on, so its mask is the last one rotated by the same amount for p.
+__device__ __forceinline__ uint32_t rot_step(uint32_t x) { return __umul24(x, 0xFFFFF8u); }
+__device__ __forceinline__ uint32_t rotate_by(uint32_t m, uint32_t step) {
+  return __builtin_amdgcn_alignbit(m, m, step);
+}
+__device__ __forceinline__ uint32_t byte_mask(uint32_t mc, uint32_t off) { return rotate_by(mc, rot_step(off)); }
+// mc of class c from a prime's 8-byte NOTBIT row (v_perm: bytes 1-3 0xFF)
+__device__ __forceinline__ uint32_t row_mask(uint64_t row, int c) {
+  return __builtin_amdgcn_perm((uint32_t)(row >> 32), (uint32_t)row, 0x0D0D0D00u | (uint32_t)c);
+}
+__device__ __forceinline__ uint32_t mask_word(uint32_t nb) { return 0xFFFFFF00u | nb; }
+
 // x mod p for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
 __device__ __forceinline__ uint32_t mod_u32(uint32_t x, uint32_t p) {
   uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)p));
@@ -361,17 +378,16 @@ __device__ __forceinline__ uint32_t mod_u32(uint32_t x, uint32_t p) {
   return (uint32_t)r;
 }
 
-// Clear bit `bit` of bytes j, j+p, j+2p, ... below `end` (segment byte
-// offsets; base = LDS byte address of the segment).  The byte-in-dword
-// pattern repeats every 4 crossings, so the four dword deltas and masks are
-// loop invariants: one add per crossing plus the ds_and_b32.
-__device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t p, uint32_t end, uint32_t bit) {
+// Clear the class bit (mask word mc) of bytes j, j+p, j+2p, ... below `end`
+// (segment byte offsets; base = LDS byte address of the segment).  The
+// byte-in-dword pattern repeats every 4 crossings, so the four dword deltas
+// and masks are loop invariants: one add per crossing plus the ds_and_b32.
+__device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t p, uint32_t end, uint32_t mc) {
   const uint32_t j0 = j & 3;
   const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
-  const uint32_t m0 = ~(1u << (j0 * 8 + bit));
-  const uint32_t m1 = ~(1u << (((j0 + p) & 3) * 8 + bit));
-  const uint32_t m2 = ~(1u << (((j0 + 2 * p) & 3) * 8 + bit));
-  const uint32_t m3 = ~(1u << (((j0 + 3 * p) & 3) * 8 + bit));
+  const uint32_t rp = rot_step(p);
+  const uint32_t m0 = byte_mask(mc, j0);
+  const uint32_t m1 = rotate_by(m0, rp), m2 = rotate_by(m1, rp), m3 = rotate_by(m2, rp);
   uint32_t a = base + (j & ~3u);
   const int32_t lim = (int32_t)(base + end) - (int32_t)(j0 + 3 * p);  // a full group fits while a < lim
   const uint32_t step = 4 * p;
@@ -388,14 +404,14 @@ __device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t 
   if (jt + 2 * p < end) lds_and_at(a + d2, m2);
 }
 
-// Clear bit `bit` of cnt bytes s, s+p, s+2p, ... (as cross_class, counted).
-__device__ __forceinline__ void cross_run(uint32_t base, uint32_t s, uint32_t p, uint32_t cnt, uint32_t bit) {
+// Clear the class bit (mask word mc) of cnt bytes s, s+p, s+2p, ... (as
+// cross_class, counted).
+__device__ __forceinline__ void cross_run(uint32_t base, uint32_t s, uint32_t p, uint32_t cnt, uint32_t mc) {
   const uint32_t j0 = s & 3;
   const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
-  const uint32_t m0 = ~(1u << (j0 * 8 + bit));
-  const uint32_t m1 = ~(1u << (((j0 + p) & 3) * 8 + bit));
-  const uint32_t m2 = ~(1u << (((j0 + 2 * p) & 3) * 8 + bit));
-  const uint32_t m3 = ~(1u << (((j0 + 3 * p) & 3) * 8 + bit));
+  const uint32_t rp = rot_step(p);
+  const uint32_t m0 = byte_mask(mc, j0);
+  const uint32_t m1 = rotate_by(m0, rp), m2 = rotate_by(m1, rp), m3 = rotate_by(m2, rp);
   uint32_t a = base + (s & ~3u);
   const uint32_t step = 4 * p;
   for (uint32_t g = cnt >> 2; g; --g) {
@@ -435,10 +451,9 @@ __device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint3
   return offl;
 }
 
-// Clear bit `bit` of segment byte off.
-__device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t bit) {
-  const uint32_t a = base + off;
-  lds_and_at(a & ~3u, ~(1u << (((a & 3) << 3) | bit)));
+// Clear the class bit (mask word mc) of segment byte off.
+__device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t mc) {
+  lds_and_at(base + (off & ~3u), byte_mask(mc, off));
 }
 
 // The crossings of a prime p >= nbytes (at most one per class), every class
@@ -453,7 +468,7 @@ __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, ui
   for (int c = 0; c < 8; ++c) {
     const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
     const uint32_t off = min(cc + prk, cc - rk);
-    if (off < nbytes) cross_one(base, off, (uint32_t)((bits >> (8 * c)) & 7));
+    if (off < nbytes) cross_one(base, off, row_mask(bits, c));
   }
 }
 
@@ -469,7 +484,7 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
   const bool low = p >= plow;  // the p*p rule can bite
-  const uint32_t rot = (p & 3) << 3;
+  const uint32_t rp = rot_step(p);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
@@ -479,17 +494,17 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
       const uint64_t fb = (uint64_t)p * m / 30;
       if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
     }
-    const uint32_t bit = (uint32_t)((bits >> (8 * c)) & 7);
+    const uint32_t mc = row_mask(bits, c);
     if (LONG) {
-      if (off < nbytes) cross_class(base, off, p, nbytes, bit);
+      if (off < nbytes) cross_class(base, off, p, nbytes, mc);
     } else if (off < nbytes) {
       // one crossing per step: off advances by p, so the byte within the
       // dword advances by p mod 4 and the mask rotates by 8 (p mod 4) bits
-      uint32_t m = ~(1u << (((off & 3) << 3) | bit));
+      uint32_t m = byte_mask(mc, off);
       do {
         lds_and_at(base + (off & ~3u), m);
         off += p;
-        m = __builtin_rotateleft32(m, rot);
+        m = rotate_by(m, rp);
       } while (off < nbytes);
     }
   }
@@ -569,7 +584,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
         const uint32_t K = div_u32(nbytes - 1 - j, p0) + 1;  // crossings j + k*p < nbytes
         const uint32_t m = ((K + 127) >> 7) | 1u;
         const uint32_t kb = hl * 4 * m;
-        if (kb < K) cross_run(base, j + kb * p0, p0, min(4 * m, K - kb), tab[64 + r0 * 8 + c]);
+        if (kb < K) cross_run(base, j + kb * p0, p0, min(4 * m, K - kb), mask_word(tab[64 + r0 * 8 + c]));
       }
     } else if (it < e1) {
       // tier 1: one prime per wave, lanes = 8 classes x 8 sub-ranges
@@ -586,7 +601,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
           const uint32_t back = mod_u32(lo - j, p1);
           j = lo + (back ? p1 - back : 0u);
         }
-        if (j < hi) cross_class(base, j, p1, hi, tab[64 + r1 * 8 + c]);
+        if (j < hi) cross_class(base, j, p1, hi, mask_word(tab[64 + r1 * 8 + c]));
       }
     } else {
       // tier 3: one prime per lane, at most four crossings per class
