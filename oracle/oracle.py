@@ -48,7 +48,9 @@ def lib():
     global _lib
     if _lib is None:
         so = _HERE / "liboracle.so"
-        if _stale(so):
+        if os.environ.get("VOSS_ORACLE_LIB"):  # e.g. the sanitizer build (make -C oracle san)
+            so = Path(os.environ["VOSS_ORACLE_LIB"])
+        elif _stale(so):
             build()
         L = ctypes.CDLL(str(so))
         u64, u32, i32 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int

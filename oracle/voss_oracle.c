@@ -345,7 +345,7 @@ int orc_full_stats(uint64_t N, uint64_t cp_interval, uint32_t large_threshold,
     out->max_gap = rmax;
     out->n_large = lg.n;
     out->n_records = recs.n;
-    qsort(lg.v, lg.n, sizeof(orc_gap_t), cmp_large);
+    if (lg.n > 1) qsort(lg.v, lg.n, sizeof(orc_gap_t), cmp_large);  /* lg.v is NULL while empty */
     if (large) for (uint64_t i = 0; i < lg.n && i < large_cap; ++i) large[i] = lg.v[i];
     if (records) for (uint64_t i = 0; i < recs.n && i < rec_cap; ++i) records[i] = recs.v[i];
     if (cp_out && bucket) {

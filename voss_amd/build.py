@@ -1,6 +1,6 @@
 """Build libvoss.so (host C++ engine + gfx950 HIP kernels) in-tree.
 
-    python voss_amd/build.py
+    python voss_amd/build.py [--sanitize]
 
 (run as a script: importing the package itself needs the built library)
 
@@ -31,31 +31,41 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 
-def _compile(src: str) -> Path:
-    obj = BUILD / (src + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+# host code only (never the device code): AddressSanitizer and UBSan
+SANITIZE = ["-Xarch_host", "-fsanitize=address,undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def _compile(src: str, bdir: Path = BUILD, extra=()) -> Path:
+    obj = bdir / (src + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
     return obj
 
 
-def build(verbose: bool = False) -> Path:
-    BUILD.mkdir(exist_ok=True)
+def build(verbose: bool = False, sanitize: bool = False) -> Path:
+    """Build voss_amd/libvoss.so; sanitize=True instead builds
+    _build/san/libvoss.so with the host code under AddressSanitizer and
+    UBSan (load it with VOSS_LIBRARY_PATH and libasan preloaded)."""
+    bdir = BUILD / "san" if sanitize else BUILD
+    out = bdir / "libvoss.so" if sanitize else OUT
+    extra = SANITIZE if sanitize else []
+    bdir.mkdir(parents=True, exist_ok=True)
     srcs = [s for s in SOURCES if (CSRC / s).exists()]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+        objs = list(ex.map(lambda src: _compile(src, bdir, extra), srcs))
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *extra, *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, sanitize="--sanitize" in sys.argv[1:])
     sys.exit(0)
