@@ -293,6 +293,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=8000)
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="print the LDS rate probe (ds_and/ds_read, conflict-free/random) as one JSON line and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the sieve shards by "
@@ -304,6 +306,18 @@ def main():
     import voss_amd
     st = voss_amd.stats
     dev = st.device_info()
+    if args.lds_probe:
+        names = ["ds_and_b32 conflict-free", "ds_and_b32 random", "ds_read_b32 conflict-free", "ds_read_b32 random"]
+        st.lds_probe(0, 1000)  # warm
+        out = {}
+        for k, nm in enumerate(names):
+            runs = [st.lds_probe(k) for _ in range(3)]
+            best = max(runs, key=lambda r: r["ops_per_clk_cu"])
+            best["ops_per_clk_cu_at_2p4ghz"] = best["ops_per_clk_cu"] * best["clock_hz"] / 2.4e9
+            out[nm] = best
+        print(json.dumps({"lds_probe": out, "device": dev, "accesses_per_thread": 8 * 20000,
+                          "note": "lane-operations per clock per CU; one 1024-thread workgroup per CU, 128 KiB LDS"}))
+        return
     for _ in range(args.warmup):
         st.full_stats(N, cp_interval=10**9, as_numpy=True)
     st.synchronize()

@@ -191,6 +191,14 @@ int voss_amd_set_config_for(int kind, uint32_t seg_bytes, int block, int wgs_per
 int voss_amd_get_config_for(int kind, uint32_t* seg_bytes, int* block, int* wgs_per_cu, uint32_t* tier1_p);
 /* Device time of the last engine call (HIP events on the engine stream). */
 int voss_amd_last_kernel_ms(double* sieve_ms, double* stitch_ms, double* total_ms, uint64_t* segments);
+/* Measurement aid (no counterpart in the reference): the LDS rate this
+ * device sustains for the sieve's access kinds, as lane-operations per clock
+ * per CU at the shader clock measured during the run.  One 1024-thread
+ * workgroup per CU over a 128 KiB LDS array, 8 * iters accesses per thread;
+ * kind 0: ds_and_b32 at conflict-free addresses (consecutive dwords across
+ * the lanes), 1: ds_and_b32 at pseudo-random addresses, 2: ds_read_b32
+ * conflict-free, 3: ds_read_b32 random. */
+int voss_amd_lds_probe(int kind, uint32_t iters, double* ops_per_clk_cu, double* clock_hz, double* ms);
 /* Compute units and architecture name of the device in use. */
 int voss_amd_device_info(int* num_cus, char* arch, int arch_len);
 int voss_amd_synchronize(void);

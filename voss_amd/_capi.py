@@ -143,6 +143,8 @@ _SIGNATURES = {
     "voss_amd_batch_ranges": ([_pu64, _pu64, _u64, _P(AmdRange)], _int),
     "voss_amd_count_range": ([_u64, _u64, _pu64], _int),
     "voss_amd_count_range_noalloc": ([_u64, _u64, ctypes.c_void_p, ctypes.c_void_p], _int),
+    "voss_amd_lds_probe": ([_int, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_double)], _int),
     "voss_amd_is_prime_batch": ([_pu64, _u64, _pu8], _int),
     "voss_amd_next_prime_batch": ([_pu64, _u64, _pu64], _int),
     "voss_amd_prev_prime_batch": ([_pu64, _u64, _pu64], _int),

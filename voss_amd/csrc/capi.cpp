@@ -645,6 +645,19 @@ extern "C" int voss_amd_count_range_noalloc(uint64_t a, uint64_t b, uint64_t* d_
   });
 }
 
+extern "C" int voss_amd_lds_probe(int kind, uint32_t iters, double* ops_per_clk_cu, double* clock_hz, double* ms) {
+  if (!ops_per_clk_cu || !clock_hz || !ms) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
+  return guarded([&] {
+    Engine& e = eng();
+    std::lock_guard<std::mutex> lk(e.mu);
+    const Engine::LdsProbe r = e.lds_probe(kind, iters);
+    *ops_per_clk_cu = r.ops_per_clk_cu;
+    *clock_hz = r.clock_hz;
+    *ms = r.ms;
+    return VOSS_OK;
+  });
+}
+
 extern "C" int voss_amd_is_prime_batch(const uint64_t* x, uint64_t n, uint8_t* out) {
   if ((!x || !out) && n) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
   return guarded([&] {

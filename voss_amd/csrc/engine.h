@@ -159,6 +159,13 @@ class Engine {
 
   KernelTimes last_times() const { return times_; }
 
+  // LDS rate probe (launch_lds_probe): lane-operations per clock per CU at
+  // the clock measured during the run, that clock, and the kernel time.
+  struct LdsProbe {
+    double ops_per_clk_cu = 0, clock_hz = 0, ms = 0;
+  };
+  LdsProbe lds_probe(int kind, uint32_t iters);
+
   // Prime count of [a, b] written to d_out[0] (device memory) by kernels
   // enqueued on stream st, with no allocation, no copy and no wait: every
   // buffer beyond d_out is the fixed scratch array.  Calls that overlap on

@@ -406,6 +406,27 @@ void Engine::count_range_noalloc(uint64_t a, uint64_t b, unsigned long long* d_o
   hip_check(launch_sum_counts(sums, nseg, seeds, d_out, st), "launch(sum_counts)");
 }
 
+Engine::LdsProbe Engine::lds_probe(int kind, uint32_t iters) {
+  if (kind < 0 || kind > 3) throw Error(kErrArg, "probe kind must be 0..3");
+  if (iters == 0 || iters > (1u << 24)) throw Error(kErrArg, "probe iterations must be in [1, 2^24]");
+  DevBuf d(64);
+  hip_check(hipMemsetAsync(d.p, 0, 64, stream_), "memset");
+  Event e0, e1;
+  hip_check(hipEventRecord(e0.e, stream_), "hipEventRecord");
+  hip_check(launch_lds_probe(kind, (uint32_t)ncu_, iters, d.as<unsigned long long>(), d.as<uint32_t>() + 8, stream_),
+            "launch(lds_probe)");
+  hip_check(hipEventRecord(e1.e, stream_), "hipEventRecord");
+  unsigned long long clk[2] = {0, 0};
+  hip_check(hipMemcpyAsync(clk, d.p, 16, hipMemcpyDeviceToHost, stream_), "D2H");
+  hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  LdsProbe R;
+  R.ms = elapsed(e0, e1);
+  R.clock_hz = clk[1] ? (double)clk[0] / ((double)clk[1] / 100e6) : 0.0;  // real-time counter: 100 MHz
+  const double ops = (double)ncu_ * 1024.0 * 8.0 * (double)iters;
+  if (R.ms > 0 && R.clock_hz > 0) R.ops_per_clk_cu = ops / (R.ms / 1e3) / (double)ncu_ / R.clock_hz;
+  return R;
+}
+
 uint64_t Engine::count_range(uint64_t lo, uint64_t hi) {
   if (hi < lo) return 0;
   uint64_t c = 0;

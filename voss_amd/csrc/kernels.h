@@ -64,6 +64,11 @@ hipError_t launch_batch_stitch(const BatchStitchArgs& A, uint32_t nq_cap, hipStr
 // workgroup per request, n_cap workgroups launched.
 hipError_t launch_select(int block, const SieveArgs& A, const uint32_t* segs, const uint32_t* ranks,
                          unsigned long long* out, const uint32_t* n_dev, uint32_t n_cap, hipStream_t st);
+// LDS rate probe (kind: bit 0 random addresses, bit 1 reads instead of ANDs):
+// `blocks` 1024-thread workgroups, 8 * iters accesses per thread; clocks[0] /
+// clocks[1] = shader-clock / 100 MHz real-time ticks of block 0.
+hipError_t launch_lds_probe(int kind, uint32_t blocks, uint32_t iters, unsigned long long* clocks, uint32_t* sink,
+                            hipStream_t st);
 // Zero n 32-bit words (graph-safe fill, no memset node).
 hipError_t launch_zero(uint32_t* p, uint64_t n, hipStream_t st);
 // out[0] = add + sums[0].count + ... + sums[n-1].count (one workgroup).

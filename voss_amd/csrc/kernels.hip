@@ -1695,6 +1695,64 @@ hipError_t launch_list_write(const uint8_t* bm, uint64_t nbytes, uint64_t k_base
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- LDS probe
+// The LDS rate this device sustains for the sieve's access kinds: one
+// 1024-thread workgroup per CU over a 128 KiB LDS array (every index masked
+// to it), 8 accesses per loop turn.  KIND bit 0: pseudo-random addresses (a
+// per-lane linear congruential sequence) instead of consecutive dwords
+// across the lanes (conflict-free); bit 1: ds_read_b32 instead of ds_and_b32.
+// Block 0's thread 0 also reads the shader clock and the 100 MHz real-time
+// counter at both ends, so the host can state the clock it ran at.
+constexpr uint32_t kProbeWords = 32768;
+template <int KIND>
+__global__ void __launch_bounds__(1024) lds_probe_kernel(uint32_t iters, unsigned long long* clocks,
+                                                         uint32_t* sink) {
+  extern __shared__ uint32_t lds[];
+  for (uint32_t i = threadIdx.x; i < kProbeWords; i += 1024) lds[i] = 0xFFFFFFFFu;
+  __syncthreads();
+  unsigned long long t0 = 0, r0 = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    t0 = __builtin_amdgcn_s_memtime();
+    r0 = __builtin_amdgcn_s_memrealtime();
+  }
+  uint32_t x = threadIdx.x * 2654435761u + blockIdx.x * 40503u + 1u;
+  uint32_t acc = 0;
+  for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      uint32_t idx;
+      if (KIND & 1) {
+        x = x * 1664525u + 1013904223u;
+        idx = x >> 17;  // 15 bits: < kProbeWords
+      } else {
+        idx = (threadIdx.x + 1024u * k + 8192u * it) & (kProbeWords - 1);
+      }
+      // volatile LDS reads (dynamic LDS starts at address 0): every read issued, none merged
+      if (KIND & 2) acc ^= ((volatile lds_u32*)(uintptr_t)0)[idx];
+      else __hip_atomic_fetch_and(lds + idx, ~(1u << k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    clocks[0] = __builtin_amdgcn_s_memtime() - t0;
+    clocks[1] = __builtin_amdgcn_s_memrealtime() - r0;
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // keeps the reads
+}
+
+hipError_t launch_lds_probe(int kind, uint32_t blocks, uint32_t iters, unsigned long long* clocks, uint32_t* sink,
+                            hipStream_t st) {
+  const uint32_t lds = kProbeWords * 4;
+  switch (kind) {
+    case 0: hipLaunchKernelGGL((lds_probe_kernel<0>), dim3(blocks), dim3(1024), lds, st, iters, clocks, sink); break;
+    case 1: hipLaunchKernelGGL((lds_probe_kernel<1>), dim3(blocks), dim3(1024), lds, st, iters, clocks, sink); break;
+    case 2: hipLaunchKernelGGL((lds_probe_kernel<2>), dim3(blocks), dim3(1024), lds, st, iters, clocks, sink); break;
+    case 3: hipLaunchKernelGGL((lds_probe_kernel<3>), dim3(blocks), dim3(1024), lds, st, iters, clocks, sink); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t prepare_kernels(uint32_t lds_bytes) {
   const void* fns[] = {
       (const void*)sieve_kernel<kModeCount, 1024>, (const void*)sieve_kernel<kModeStats, 1024>,
@@ -1702,7 +1760,9 @@ hipError_t prepare_kernels(uint32_t lds_bytes) {
       (const void*)sieve_kernel<kModeStats, 512>,  (const void*)sieve_kernel<kModeBatch, 512>,
       (const void*)sieve_kernel<kModeBitmap, 1024>, (const void*)sieve_kernel<kModeBitmap, 512>,
       (const void*)rescan_kernel<1024>,            (const void*)rescan_kernel<512>,
-      (const void*)select_kernel<1024>,            (const void*)select_kernel<512>};
+      (const void*)select_kernel<1024>,            (const void*)select_kernel<512>,
+      (const void*)lds_probe_kernel<0>,            (const void*)lds_probe_kernel<1>,
+      (const void*)lds_probe_kernel<2>,            (const void*)lds_probe_kernel<3>};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;

@@ -201,6 +201,15 @@ def last_kernel_ms() -> dict:
     return {"sieve_ms": a.value, "stitch_ms": b.value, "total_ms": c.value, "segments": s.value}
 
 
+def lds_probe(kind: int, iters: int = 20000) -> dict:
+    """LDS lane-operations per clock per CU for one access kind (0: ds_and
+    conflict-free, 1: ds_and random, 2: ds_read conflict-free, 3: ds_read
+    random), the shader clock measured meanwhile and the kernel time."""
+    r, c, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    _capi._check(_L.voss_amd_lds_probe(kind, iters, ctypes.byref(r), ctypes.byref(c), ctypes.byref(m)), _ERRMAP)
+    return {"ops_per_clk_cu": r.value, "clock_hz": c.value, "ms": m.value}
+
+
 def device_info() -> dict:
     n = ctypes.c_int()
     name = ctypes.create_string_buffer(64)
