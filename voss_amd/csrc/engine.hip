@@ -476,7 +476,12 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   const uint64_t nseg64 = (k_end + S - 1) / S;
   if (nseg64 > 0x7FFFFFF0ull) throw Error(kErrArg, "N too large for the segment size");
   const uint32_t nseg = (uint32_t)nseg64;
-  const uint32_t large_cap = 1u << 20, rec_cap = 4096, rescan_cap = 4096, rout_cap = 16384;
+  // Record lists cannot overflow: every boundary record, every listed wave
+  // range and every rescanned record strictly raises a running maximum that
+  // the stitch's block scan carries across its threads, and the listed
+  // values are 16-bit (gaps and per-wave bounds below 65536), so each list
+  // holds fewer than 2^16 entries (1 MiB each).
+  const uint32_t large_cap = 1u << 20, rec_cap = 1u << 16, rescan_cap = 1u << 16, rout_cap = 1u << 16;
 
   // Progress reports per reference segment (context.cu:260-269, 433-436):
   // SEG_NUM = min(ceil(N/30)*30 + 10, 5e10 + 10) integers, NUM_SEG =
@@ -645,7 +650,8 @@ SweepStats Engine::sweep_stats(uint64_t N, uint64_t cp_interval, uint32_t large_
   const unsigned long long* tot = reinterpret_cast<const unsigned long long*>(hs + o_tot);
   const unsigned long long* dh = reinterpret_cast<const unsigned long long*>(hs + o_hist);
   const unsigned long long* bk = reinterpret_cast<const unsigned long long*>(hs + o_bk);
-  if (cnts[2] > rescan_cap) throw Error(7, "record rescan list overflow");
+  if (cnts[1] > rec_cap || cnts[2] > rescan_cap || cnts[3] > rout_cap)
+    throw Error(7, "internal: record list overflow");  // excluded by the bound above
   const uint32_t n_large = std::min(cnts[0], large_cap), n_rec = std::min(cnts[1], rec_cap),
                  n_rout = std::min(cnts[3], rout_cap);
   auto take = [&](size_t off, const DevBuf& src, uint32_t n) {
