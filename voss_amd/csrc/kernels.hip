@@ -883,23 +883,28 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     const uint32_t prev0 = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
     carry = max(carry, step_last);
     const uint32_t vb = 240u * boff;
-    uint32_t prev = prev0 ? 8u * prev0 : vb, lm = 0;
+    // np = vb - 8 * (the last prime so far), or 0 before the wave's first
+    uint32_t np = prev0 ? vb - 8u * prev0 : 0u, lm = 0;
     const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
       // 8 * W30 of the first and last set bit from two byte tables (v_perm):
-      // selector 8 (x = 0) reads 0x00 from the sign of table byte 1 (56)
-      const uint32_t wf8 = __builtin_amdgcn_perm(kW30x8Hi, kW30x8Lo, __builtin_ctz(x | 0x100u)) & 0xFFu;
-      const uint32_t wl8 = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, __builtin_clz(x << 24)) & 0xFFu;
-      const uint32_t bj = vb + 240u * j;
-      const uint32_t gap8 = bj + wf8 - prev;
+      // selector 8 (x = 0) reads 0x00 from the sign of table byte 1 (56).
+      // The selector's upper bytes are 0, so the result's upper bytes repeat
+      // table byte 0: 0x08080800 + 8 * W30 (first) and 0xE8E8E800 + 8 * W30
+      // (last); the offsets are folded into the constants below.
+      const uint32_t t = __builtin_ctz(x | 0x100u);
+      const uint32_t pf = __builtin_amdgcn_perm(kW30x8Hi, kW30x8Lo, t);
+      const uint32_t gap8 = pf + np + (240u * j - 0x08080800u);
       lm = max(lm, gap8);
       if (x) {
         lds_add((lds_u32*)(H8 + 2 * min(gap8, kCap8)), 1u);
-        prev = bj + wl8;
+        const uint32_t pl = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, __builtin_clz(x << 24));
+        np = (0xE8E8E800u - 240u * j) - pl;
       }
-      if (__builtin_popcount(x) > 1) lds_add((lds_u32*)((lds_u8*)(uintptr_t)(4 * x) + kLdsByteHist * 4), 1u);
+      // two or more set bits: x is not its lowest set bit (1 << t; 0 for x = 0)
+      if (x != ((1u << t) & 0xFFu)) lds_add((lds_u32*)((lds_u8*)(uintptr_t)(4 * x) + kLdsByteHist * 4), 1u);
     }
     if (prev0 == 0 && (q.x | q.y | q.z | q.w) != 0) {
       const uint32_t vfirst = chunk_first(q, boff);
