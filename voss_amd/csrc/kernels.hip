@@ -362,7 +362,12 @@ __device__ __forceinline__ uint32_t rot_step(uint32_t x) { return __umul24(x, 0x
 __device__ __forceinline__ uint32_t rotate_by(uint32_t m, uint32_t step) {
   return __builtin_amdgcn_alignbit(m, m, step);
 }
-__device__ __forceinline__ uint32_t byte_mask(uint32_t mc, uint32_t off) { return rotate_by(mc, rot_step(off)); }
+// (alignbyte rotates right by 8 (-off mod 4) bits: the same rotation from two
+// full-rate instructions, where the shift off * (2^24 - 8) of an off not
+// known to fit 24 bits becomes a quarter-rate 32-bit multiply)
+__device__ __forceinline__ uint32_t byte_mask(uint32_t mc, uint32_t off) {
+  return __builtin_amdgcn_alignbyte(mc, mc, 0u - off);
+}
 // mc of class c from a prime's 8-byte NOTBIT row (v_perm: bytes 1-3 0xFF)
 __device__ __forceinline__ uint32_t row_mask(uint64_t row, int c) {
   return __builtin_amdgcn_perm((uint32_t)(row >> 32), (uint32_t)row, 0x0D0D0D00u | (uint32_t)c);
