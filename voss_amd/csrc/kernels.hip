@@ -354,17 +354,17 @@ __device__ __forceinline__ uint32_t seg_base_addr(uint32_t*) {
 
 // Crossing masks.  mc = 0xFFFFFF00 | nb, nb the byte ~(1 << bit) of a class
 // (table row NOTBIT).  The mask that clears the class bit of the byte at
-// segment offset off is mc rotated left by 8 (off mod 4) bits: alignbit
-// rotates right by its shift's low five bits, and off * (2^24 - 8) has low
-// five bits -8 off mod 32 (off < 2^24).  A prime's next crossing is p bytes
-// on, so its mask is the last one rotated by the same amount for p.
+// segment offset off is mc rotated left by 8 (off mod 4) bits, i.e. right by
+// 8 (-off mod 4): one alignbyte on -off (a shift off * (2^24 - 8) for
+// alignbit would be lowered to a quarter-rate 32-bit multiply, off not being
+// known to fit 24 bits).  A prime's next crossing is p bytes on, so its mask
+// is the last one rotated by the same amount for p: alignbit rotates right
+// by its shift's low five bits, and p * (2^24 - 8) (a 24-bit multiply,
+// p < 2^24) has low five bits -8 p mod 32.
 __device__ __forceinline__ uint32_t rot_step(uint32_t x) { return __umul24(x, 0xFFFFF8u); }
 __device__ __forceinline__ uint32_t rotate_by(uint32_t m, uint32_t step) {
   return __builtin_amdgcn_alignbit(m, m, step);
 }
-// (alignbyte rotates right by 8 (-off mod 4) bits: the same rotation from two
-// full-rate instructions, where the shift off * (2^24 - 8) of an off not
-// known to fit 24 bits becomes a quarter-rate 32-bit multiply)
 __device__ __forceinline__ uint32_t byte_mask(uint32_t mc, uint32_t off) {
   return __builtin_amdgcn_alignbyte(mc, mc, 0u - off);
 }
