@@ -479,22 +479,24 @@ __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, ui
 
 // Every crossing of p (rk = k0 mod p) in the segment, class by class, from
 // two packed 64-bit LDS rows: off = min(cc + p - rk, cc - rk), the p*p rule
-// applied where it can bite.  LONG walks each class with the unrolled
-// cross_class loop (many crossings per class), otherwise one at a time.
-template <bool LONG = false>
+// applied if LOW.  LONG walks each class with the unrolled cross_class loop
+// (many crossings per class), otherwise one at a time.  Callers pass LOW for
+// a whole wave when the rule can bite for any of its lanes (p >= plow): the
+// check leaves off unchanged where p < plow, whose p * (p + 30) <= 30 k0, so
+// the wave takes one scalar branch instead of an exec-masked one per class.
+template <bool LONG, bool LOW>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
                                                     uint32_t nbytes, const uint8_t* tab, uint32_t plow) {
   const uint32_t q30 = p / 30;
   const uint32_t prk = p - rk;
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
-  const bool low = p >= plow;  // the p*p rule can bite
   const uint32_t rp = rot_step(p);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
     uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
-    if (low) {
+    if (LOW) {
       const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
       const uint64_t fb = (uint64_t)p * m / 30;
       if (k0 + off < fb) off = (fb - k0 > 0xFFFFFFFFull) ? 0xFFFFFFFFu : (uint32_t)(fb - k0);
@@ -572,7 +574,11 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     if (it < e2) {
       // tier 2: one prime per lane, each class walked by the unrolled loop
-      if (e) cross_prime_classes<true>(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
+      if (e) {
+        const uint32_t rk = mod_u64(k0, p);
+        if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<true, true>(base, p, r, rk, k0, nbytes, tab, plow);
+        else cross_prime_classes<true, false>(base, p, r, rk, k0, nbytes, tab, plow);
+      }
     } else if (it < e0) {
       // tier 0: item = (prime, class pair); half-wave h takes class 2q + h.
       // The class's K crossings are split into runs of 4m (m odd, 32 runs
@@ -610,7 +616,11 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       }
     } else {
       // tier 3: one prime per lane, at most four crossings per class
-      if (e) cross_prime_classes(base, p, r, mod_u64(k0, p), k0, nbytes, tab, plow);
+      if (e) {
+        const uint32_t rk = mod_u64(k0, p);
+        if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<false, true>(base, p, r, rk, k0, nbytes, tab, plow);
+        else cross_prime_classes<false, false>(base, p, r, rk, k0, nbytes, tab, plow);
+      }
     }
     it = nit;
     e = ne;
@@ -655,7 +665,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       const uint32_t p = e4 & 0xFFFFFFu, r = e4 >> 24;
       const uint32_t rk = mod_u64(k0, p);
       if (p >= plow) {  // the p*p rule can bite: general path
-        cross_prime_classes(base, p, r, rk, k0, nbytes, tab, plow);
+        cross_prime_classes<false, true>(base, p, r, rk, k0, nbytes, tab, plow);
         continue;
       }
       cross_classes_once(base, p, r, rk, nbytes, tab);
