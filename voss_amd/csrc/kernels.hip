@@ -229,7 +229,7 @@ __device__ void init_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   const uint32_t nfull = g.nbytes >> 8;
   const uint32_t first = (g.k0 < 4 && nfull > 0) ? 1u : 0u;  // else the thread pass covers chunk 0
   {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // wave 0 may still be merging the previous segment: waves 1.. copy
     if (wave > 0)
       for (uint32_t blk = first + wave - 1; blk < nfull; blk += BS / 64 - 1)
@@ -834,7 +834,7 @@ __device__ void chunk_big_gaps(const SieveArgs& A, uint32_t* lds, uint4 q, uint3
 template <int MODE, int BS>
 __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
   constexpr int NW = BS / 64;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
   const uint32_t copy = lane & 7;
   const uint32_t wave_bytes = A.seg_bytes / NW;
@@ -1235,7 +1235,7 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
 // scratch; every thread must call it (it has barriers).
 template <typename Op>
 __device__ uint64_t block_exclusive_scan(uint64_t v, uint64_t ident, Op op, unsigned long long* tot) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   auto up = [](uint64_t x, int d) -> uint64_t {
     const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)x, d, 64);
     const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(x >> 32), d, 64);
@@ -1400,7 +1400,7 @@ template <int BS>
 __global__ void __launch_bounds__(BS) rescan_kernel(SieveArgs A, RescanArgs R) {
   extern __shared__ uint32_t lds[];
   constexpr int NW = BS / 64;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   init_tables(lds);
   __syncthreads();
   const uint32_t nitems = min(*R.n_items, R.items_cap);  // same value in every thread
@@ -1510,7 +1510,7 @@ __global__ void __launch_bounds__(BS) select_kernel(SieveArgs A, const uint32_t*
                                                     unsigned long long* out, const uint32_t* n_dev) {
   extern __shared__ uint32_t lds[];
   constexpr int NW = BS / 64;
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (blockIdx.x >= *n_dev) return;  // uniform per workgroup
   init_tables(lds);
   __syncthreads();
@@ -1673,7 +1673,7 @@ __global__ void __launch_bounds__(256) list_count_kernel(const uint8_t* bm, uint
 __global__ void __launch_bounds__(256) list_write_kernel(const uint8_t* bm, uint64_t nbytes, uint64_t k_base,
                                                          const unsigned long long* block_off, uint64_t* out) {
   __shared__ uint32_t wsum[4];
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t off = (uint64_t)blockIdx.x * kListBlock + 16 * threadIdx.x;
   uint4 q = make_uint4(0, 0, 0, 0);
   if (off < nbytes) q = *reinterpret_cast<const uint4*>(bm + off);
