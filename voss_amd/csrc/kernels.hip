@@ -49,6 +49,8 @@ __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
 }
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 __device__ __forceinline__ void lds_add(lds_u32* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -744,9 +746,9 @@ __device__ __forceinline__ uint32_t prev_in_step(uint32_t lastv, uint32_t lane, 
   const bool dense = __ballot(lastv == 0) == 0ull;
   if (dense) {
     // every lane holds a prime: the predecessor is simply lane-1's last
-    step_last = __shfl((int)lastv, 63, 64);
-    uint32_t up = (uint32_t)__shfl_up((int)lastv, 1, 64);
-    return lane ? up : 0u;
+    // (DPP wave_shr:1: lane 0 keeps the 0 passed as the old value)
+    step_last = __builtin_amdgcn_readlane(lastv, 63);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lastv, 0x138, 0xF, 0xF, false);
   }
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -872,7 +874,9 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
       else if (done == 8) __builtin_amdgcn_s_setprio(0);
     }
     const uint32_t boff = base + 16 * lane;
-    uint4 q = *reinterpret_cast<const uint4*>(seg8 + boff);
+    // (dynamic LDS starts at address 0: the segment's address is a constant)
+    const u32x4 qv = *(const lds_u32x4*)(uintptr_t)(kLdsSeg * 4 + boff);
+    uint4 q = make_uint4(qv.x, qv.y, qv.z, qv.w);
     if (apply_rp) {
       const uint4 b = bnext;
       if (base + 1024 < wend) bnext = read_pattern_chunk(RP, boff + 1024);
@@ -897,7 +901,7 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     uint32_t step_last;
     const uint32_t prev0 = max(prev_in_step(chunk_last(q, boff), lane, step_last), carry);
     carry = max(carry, step_last);
-    const uint32_t vb = 240u * boff;
+    const uint32_t vb = __umul24(240u, boff);
     // np = vb - 8 * (the last prime so far), or 0 before the wave's first
     uint32_t np = prev0 ? vb - 8u * prev0 : 0u, lm = 0;
     const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
