@@ -341,6 +341,29 @@ __device__ __forceinline__ uint4 read_pattern_chunk(const ReadPatterns& P, uint3
 __device__ __forceinline__ void and4(uint4& q, const uint4& b) {
   q.x &= b.x; q.y &= b.y; q.z &= b.z; q.w &= b.w;
 }
+// The read patterns' words for the chunk at boff, loaded but not yet
+// combined, so a load issued one step ahead is waited for only where the
+// next step uses it.
+struct PatternWords {
+  uint4 w[kReadPatterns];
+};
+__device__ __forceinline__ PatternWords load_pattern_words(const ReadPatterns& P, uint32_t boff) {
+  PatternWords W;
+#pragma unroll
+  for (int i = 0; i < kReadPatterns; ++i) {
+    const uint32_t* p = P.src[i] + (boff >> 2);
+    W.w[i] = make_uint4(p[0], p[1], p[2], p[3]);
+  }
+  return W;
+}
+__device__ __forceinline__ void and_pattern_words(uint4& q, const PatternWords& W, const ReadPatterns& P,
+                                                  uint32_t boff) {
+  uint4 v = W.w[0];
+#pragma unroll
+  for (int i = 1; i < kReadPatterns; ++i) and4(v, W.w[i]);
+  if (boff == 0) v.x |= P.head_keep;
+  and4(q, v);
+}
 
 __device__ __forceinline__ void lds_and_at(uint32_t addr, uint32_t m) {
   __hip_atomic_fetch_and((lds_u32*)(uintptr_t)addr, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -861,8 +884,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   // next step's are loaded one step ahead
   const bool apply_rp = !(MODE == kModeBatch && g.from_cache);  // uniform per workgroup
   const ReadPatterns RP = read_patterns(A, g.k0);
-  uint4 bnext = make_uint4(~0u, ~0u, ~0u, ~0u);
-  if (apply_rp && wbeg < wend) bnext = read_pattern_chunk(RP, wbeg + 16 * lane);
+  PatternWords bnext;
+#pragma unroll
+  for (int i = 0; i < kReadPatterns; ++i) bnext.w[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  if (apply_rp && wbeg < wend) bnext = load_pattern_words(RP, wbeg + 16 * lane);
   // waves further along drop in issue priority, so lagging waves catch up
   // and the walk's waves reach the barrier together
   __builtin_amdgcn_s_setprio(3);
@@ -878,9 +903,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     const u32x4 qv = *(const lds_u32x4*)(uintptr_t)(kLdsSeg * 4 + boff);
     uint4 q = make_uint4(qv.x, qv.y, qv.z, qv.w);
     if (apply_rp) {
-      const uint4 b = bnext;
-      if (base + 1024 < wend) bnext = read_pattern_chunk(RP, boff + 1024);
-      and4(q, b);
+      // combine this step's words (loaded one step ahead), then load the
+      // next step's into the same registers (the last step reloads its own)
+      and_pattern_words(q, bnext, RP, boff);
+      bnext = load_pattern_words(RP, base + 1024 < wend ? boff + 1024 : boff);
     }
     if (partial) {
       const uint32_t cb = 8 * boff;
