@@ -365,6 +365,13 @@ __device__ __forceinline__ void and_pattern_words(uint4& q, const PatternWords& 
   and4(q, v);
 }
 
+// A load that every lane makes from the same address of memory no kernel
+// writes: through the constant address space it is a scalar load (scalar
+// cache, waited for apart from the vector loads in flight).
+__device__ __forceinline__ uint32_t load_uniform(const uint32_t* p, uint32_t i) {
+  return ((const __attribute__((address_space(4))) uint32_t*)p)[i];
+}
+
 __device__ __forceinline__ void lds_and_at(uint32_t addr, uint32_t m) {
   __hip_atomic_fetch_and((lds_u32*)(uintptr_t)addr, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -593,6 +600,9 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   };
   uint32_t it = take();
   uint32_t e = lane_entry(it);
+  // the first entry is waited for here, so that inside the loop only the
+  // next item's entry is in flight and its wait falls at the loop's end
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   while (it < e3) {
     const uint32_t nit = take();
     const uint32_t ne = lane_entry(nit);
@@ -612,7 +622,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       const uint32_t t0 = it - e2;
       const uint32_t hl = lane & 31, half = lane >> 5;
       const uint32_t c = ((t0 & 3) << 1) | half;
-      const uint32_t e0i = A.primes[t0 >> 2];
+      const uint32_t e0i = load_uniform(A.primes, t0 >> 2);
       const uint32_t p0 = e0i & 0xFFFFFFu, r0 = e0i >> 24;
       const uint64_t offl = first_crossing(p0, r0, c, mod_u64(k0, p0), k0, tab);
       if (offl < nbytes) {
@@ -628,7 +638,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
       const uint32_t lo = min(sr * sub_len, nbytes);
       const uint32_t hi = min(lo + sub_len, nbytes);
-      const uint32_t e1i = A.primes[n0 + (it - e0)];
+      const uint32_t e1i = load_uniform(A.primes, n0 + (it - e0));
       const uint32_t p1 = e1i & 0xFFFFFFu, r1 = e1i >> 24;
       const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab);
       if (offl < hi) {
