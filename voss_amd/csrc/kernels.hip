@@ -498,13 +498,14 @@ __device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t 
 __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, uint32_t r, uint32_t rk,
                                                    uint32_t nbytes, const uint8_t* tab) {
   const uint32_t q30 = p / 30;
-  const uint32_t prk = p - rk;
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
-    const uint32_t off = min(cc + prk, cc - rk);
+    // cm = cc - rk (mod 2^32) as a 24-bit multiply-add on the byte's
+    // difference from rk, then off = (cc - rk) mod p = min(cm, cm + p)
+    const uint32_t cm = __umul24(q30, w30_const(c)) + ((uint32_t)((frac >> (8 * c)) & 0xFF) - rk);
+    const uint32_t off = min(cm, cm + p);
     if (off < nbytes) cross_one(base, off, row_mask(bits, c));
   }
 }
@@ -520,14 +521,13 @@ template <bool LONG, bool LOW>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
                                                     uint32_t nbytes, const uint8_t* tab, uint32_t plow) {
   const uint32_t q30 = p / 30;
-  const uint32_t prk = p - rk;
   const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
   const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
   const uint32_t rp = rot_step(p);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    const uint32_t cc = __umul24(q30, w30_const(c)) + (uint32_t)((frac >> (8 * c)) & 0xFF);
-    uint32_t off = min(cc + prk, cc - rk);  // (cc - rk) mod p, as unsigned wrap-around
+    const uint32_t cm = __umul24(q30, w30_const(c)) + ((uint32_t)((frac >> (8 * c)) & 0xFF) - rk);
+    uint32_t off = min(cm, cm + p);  // (cc - rk) mod p, as unsigned wrap-around
     if (LOW) {
       const uint64_t m = (uint64_t)p + tab[128 + r * 8 + c];
       const uint64_t fb = (uint64_t)p * m / 30;
