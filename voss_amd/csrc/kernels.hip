@@ -49,6 +49,7 @@ __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
 }
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 __device__ __forceinline__ void lds_add(lds_u32* p, uint32_t v) {
@@ -108,10 +109,12 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 
 // ---------------------------------------------------------------- LDS layout
 // u32 word offsets into dynamic LDS.
-constexpr uint32_t kLdsHist = 0;                                    // kHistBins*kHistCopies
+// The tables come first: at LDS address 0 a row's address is its index
+// times 8, within reach of ds_read2_b64's 8-bit offset fields.
+constexpr uint32_t kLdsTab = 0;                                       // 3 x 64 bytes = 48 words
+constexpr uint32_t kLdsHist = kLdsTab + 48;                           // kHistBins*kHistCopies
 constexpr uint32_t kLdsByteHist = kLdsHist + kHistBins * kHistCopies; // 256 byte values, one copy
-constexpr uint32_t kLdsTab = kLdsByteHist + 256;                      // 3 x 64 bytes = 48 words
-constexpr uint32_t kLdsMisc = kLdsTab + 48;                       // 256 words
+constexpr uint32_t kLdsMisc = kLdsByteHist + 256;                     // 256 words
 constexpr uint32_t kLdsSeg = kLdsMisc + 256;                      // segment bytes
 }  // namespace
 
@@ -475,7 +478,7 @@ __device__ __forceinline__ uint32_t div_u32(uint32_t x, uint32_t p) {
 // First crossing of class c at or after byte k0 of the global bitmap,
 // relative to k0; never below p*p (the reference rule, context.cu:354-388).
 __device__ __forceinline__ uint64_t first_crossing(uint32_t p, uint32_t r, uint32_t c, uint32_t rk, uint64_t k0,
-                                                   const uint8_t* tab) {
+                                                   const lds_u8* tab) {
   const uint32_t cc = (p / 30) * w30(c) + tab[r * 8 + c];
   int32_t off = (int32_t)cc - (int32_t)rk;
   if (off < 0) off += (int32_t)p;
@@ -496,10 +499,10 @@ __device__ __forceinline__ void cross_one(uint32_t base, uint32_t off, uint32_t 
 // The crossings of a prime p >= nbytes (at most one per class), every class
 // checked in turn with no search and no loop: off = (cc_c - rk) mod p.
 __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, uint32_t r, uint32_t rk,
-                                                   uint32_t nbytes, const uint8_t* tab) {
+                                                   uint32_t nbytes, const lds_u8* tab) {
   const uint32_t q30 = p / 30;
-  const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
-  const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+  const uint64_t frac = *(const lds_u64*)(tab + r * 8);
+  const uint64_t bits = *(const lds_u64*)(tab + 64 + r * 8);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     // cm = cc - rk (mod 2^32) as a 24-bit multiply-add on the byte's
@@ -519,10 +522,10 @@ __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, ui
 // the wave takes one scalar branch instead of an exec-masked one per class.
 template <bool LONG, bool LOW>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
-                                                    uint32_t nbytes, const uint8_t* tab, uint32_t plow) {
+                                                    uint32_t nbytes, const lds_u8* tab, uint32_t plow) {
   const uint32_t q30 = p / 30;
-  const uint64_t frac = *reinterpret_cast<const uint64_t*>(tab + r * 8);
-  const uint64_t bits = *reinterpret_cast<const uint64_t*>(tab + 64 + r * 8);
+  const uint64_t frac = *(const lds_u64*)(tab + r * 8);
+  const uint64_t bits = *(const lds_u64*)(tab + 64 + r * 8);
   const uint32_t rp = rot_step(p);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -565,7 +568,9 @@ template <int BS>
 __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g) {
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63;
-  const uint8_t* tab = reinterpret_cast<const uint8_t*>(lds + kLdsTab);
+  // the tables' LDS address is a constant (dynamic LDS starts at 0), so it
+  // folds into the instructions' offset fields
+  const lds_u8* tab = (const lds_u8*)(uintptr_t)(kLdsTab * 4);
   const uint32_t base = seg_base_addr(lds);
   const uint64_t k0 = g.k0;
   const uint32_t nbytes = g.nbytes;
