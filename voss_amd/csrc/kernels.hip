@@ -520,7 +520,7 @@ __device__ __forceinline__ void cross_classes_once(uint32_t base, uint32_t p, ui
 // a whole wave when the rule can bite for any of its lanes (p >= plow): the
 // check leaves off unchanged where p < plow, whose p * (p + 30) <= 30 k0, so
 // the wave takes one scalar branch instead of an exec-masked one per class.
-template <bool LONG, bool LOW>
+template <bool LONG, bool LOW, bool HITS = false>
 __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, uint32_t r, uint32_t rk, uint64_t k0,
                                                     uint32_t nbytes, const lds_u8* tab, uint32_t plow) {
   const uint32_t q30 = p / 30;
@@ -539,9 +539,10 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
     const uint32_t mc = row_mask(bits, c);
     if (LONG) {
       if (off < nbytes) cross_class(base, off, p, nbytes, mc);
-    } else if (off < nbytes) {
+    } else if (HITS || off < nbytes) {
       // one crossing per step: off advances by p, so the byte within the
       // dword advances by p mod 4 and the mask rotates by 8 (p mod 4) bits
+      // (HITS: every lane's p < nbytes and no p*p rule, so off < p < nbytes)
       uint32_t m = byte_mask(mc, off);
       do {
         lds_and_at(base + (off & ~3u), m);
@@ -659,7 +660,8 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (e) {
         const uint32_t rk = mod_u64(k0, p);
         if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<false, true>(base, p, r, rk, k0, nbytes, tab, plow);
-        else cross_prime_classes<false, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else if (__builtin_amdgcn_ballot_w64(p >= nbytes)) cross_prime_classes<false, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else cross_prime_classes<false, false, true>(base, p, r, rk, k0, nbytes, tab, plow);
       }
     }
     it = nit;
