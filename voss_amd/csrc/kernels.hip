@@ -422,6 +422,8 @@ __device__ __forceinline__ uint32_t mod_u32(uint32_t x, uint32_t p) {
 // (segment byte offsets; base = LDS byte address of the segment).  The
 // byte-in-dword pattern repeats every 4 crossings, so the four dword deltas
 // and masks are loop invariants: one add per crossing plus the ds_and_b32.
+// FULL: the caller knows a first group fits (j + 3p < end).
+template <bool FULL = false>
 __device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t p, uint32_t end, uint32_t mc) {
   const uint32_t j0 = j & 3;
   const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
@@ -431,12 +433,14 @@ __device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t 
   uint32_t a = base + (j & ~3u);
   const int32_t lim = (int32_t)(base + end) - (int32_t)(j0 + 3 * p);  // a full group fits while a < lim
   const uint32_t step = 4 * p;
-  while ((int32_t)a < lim) {
-    lds_and_at(a, m0);
-    lds_and_at(a + d1, m1);
-    lds_and_at(a + d2, m2);
-    lds_and_at(a + d3, m3);
-    a += step;
+  if (FULL || (int32_t)a < lim) {
+    do {
+      lds_and_at(a, m0);
+      lds_and_at(a + d1, m1);
+      lds_and_at(a + d2, m2);
+      lds_and_at(a + d3, m3);
+      a += step;
+    } while ((int32_t)a < lim);
   }
   const uint32_t jt = a - base + j0;
   if (jt < end) lds_and_at(a, m0);
@@ -538,7 +542,8 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
     }
     const uint32_t mc = row_mask(bits, c);
     if (LONG) {
-      if (off < nbytes) cross_class(base, off, p, nbytes, mc);
+      // (HITS: every lane's 4p <= nbytes and no p*p rule, so off + 3p < nbytes)
+      if (HITS || off < nbytes) cross_class<HITS>(base, off, p, nbytes, mc);
     } else if (HITS || off < nbytes) {
       // one crossing per step: off advances by p, so the byte within the
       // dword advances by p mod 4 and the mask rotates by 8 (p mod 4) bits
@@ -618,7 +623,8 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (e) {
         const uint32_t rk = mod_u64(k0, p);
         if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<true, true>(base, p, r, rk, k0, nbytes, tab, plow);
-        else cross_prime_classes<true, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else if (__builtin_amdgcn_ballot_w64(4 * p > nbytes)) cross_prime_classes<true, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else cross_prime_classes<true, false, true>(base, p, r, rk, k0, nbytes, tab, plow);
       }
     } else if (it < e0) {
       // tier 0: item = (prime, class pair); half-wave h takes class 2q + h.
