@@ -43,6 +43,20 @@ __device__ __forceinline__ uint32_t w30_const(int o) {
   return o == 0 ? 1 : o == 1 ? 7 : o == 2 ? 11 : o == 3 ? 13 : o == 4 ? 17 : o == 5 ? 19 : o == 6 ? 23 : 29;
 }
 
+// Index of the lowest set bit of byte k of w, 0xFFFFFFFF for a zero byte:
+// one v_ffbl_b32 whose SDWA source select zero-extends the byte (k is a
+// constant after unrolling, so the switch folds).
+__device__ __forceinline__ uint32_t ffbl_byte(uint32_t w, int k) {
+  uint32_t t;
+  switch (k & 3) {
+    case 0: asm("v_ffbl_b32_sdwa %0, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0" : "=v"(t) : "v"(w)); break;
+    case 1: asm("v_ffbl_b32_sdwa %0, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1" : "=v"(t) : "v"(w)); break;
+    case 2: asm("v_ffbl_b32_sdwa %0, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_2" : "=v"(t) : "v"(w)); break;
+    default: asm("v_ffbl_b32_sdwa %0, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_3" : "=v"(t) : "v"(w)); break;
+  }
+  return t;
+}
+
 // LDS atomic add without return (ds_add_u32).
 __device__ __forceinline__ void lds_add(uint32_t* p, uint32_t v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -876,9 +890,11 @@ __device__ void chunk_big_gaps(const SieveArgs& A, uint32_t* lds, uint4 q, uint3
 // u32 copies per bin).  The wheel offsets of a byte's first and last prime
 // come from v_perm byte tables (no LDS read).  A lane with no earlier prime in its wave starts from
 // its chunk's first byte; the one spurious gap that gives (chunk start to the
-// first prime) is taken back after the loop.  Zero bytes still raise the
-// chunk's running maximum `lm` by their distance from the last prime; that
-// never exceeds the gap that closes over them, so `lm` stays a bound.
+// first prime) is taken back after the loop.  The chunk's running maximum
+// `lm` takes the gaps that close in the chunk, each raised by a constant D
+// above every zero byte's distance from the last prime; a chunk without a
+// prime keeps the largest such distance instead, which never exceeds the
+// gap that closes over it, so `lm` stays a bound.
 template <int MODE, int BS>
 __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
   constexpr int NW = BS / 64;
@@ -897,6 +913,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   lds_u8* const H8 = (lds_u8*)(lds_u32*)lds + (kLdsHist + copy) * 4;
   constexpr uint32_t kCap8 = (kHistBins - 1) * 16;  // last bin collects longer gaps
   static_assert(kHistCopies == 8, "bin stride 32 bytes = 2 * (8 * gap)");
+  // D lifts every closing gap (times 8) above every zero byte's distance;
+  // H8m is H8's address less 2 D, so H8m + 2 (D + 8 gap) addresses the bin
+  constexpr uint32_t D = 0x08080801u;
+  const uint32_t H8m = (uint32_t)(uintptr_t)H8 - 2u * D;
 
   uint32_t carry = 0, wfirst = 0;
   uint32_t cnt = 0, c41 = 0, c31 = 0, maxg = 0, lcnt = 0, l41 = 0, l31 = 0;
@@ -957,23 +977,26 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
-      // 8 * W30 of the first and last set bit from two byte tables (v_perm):
-      // selector 8 (x = 0) reads 0x00 from the sign of table byte 1 (56).
+      // 8 * W30 of the first and last set bit from two byte tables (v_perm).
       // The selector's upper bytes are 0, so the result's upper bytes repeat
       // table byte 0: 0x08080800 + 8 * W30 (first) and 0xE8E8E800 + 8 * W30
-      // (last); the offsets are folded into the constants below.
-      const uint32_t t = __builtin_ctz(x | 0x100u);
+      // (last); the offsets are folded into the constants below.  A zero
+      // byte has t = 0xFFFFFFFF, every selector byte reads 0xFF, pf = -1 and
+      // g8 = np + 240 j, its distance from the last prime (times 8); a
+      // non-zero byte has g8 = D + 8 * (its gap).
+      const uint32_t t = ffbl_byte(ws[j >> 2], j);
       const uint32_t pf = __builtin_amdgcn_perm(kW30x8Hi, kW30x8Lo, t);
-      const uint32_t gap8 = pf + np + (240u * j - 0x08080800u);
-      lm = max(lm, gap8);
+      const uint32_t g8 = pf + np + (240u * j + 1u);
+      lm = max(lm, g8);
       if (x) {
-        lds_add((lds_u32*)(H8 + 2 * min(gap8, kCap8)), 1u);
+        lds_add((lds_u32*)(uintptr_t)(H8m + 2u * min(g8, kCap8 + D)), 1u);
         const uint32_t pl = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, __builtin_clz(x << 24));
         np = (0xE8E8E800u - 240u * j) - pl;
       }
       // two or more set bits: x is not its lowest set bit (1 << t; 0 for x = 0)
-      if (x != ((1u << t) & 0xFFu)) lds_add((lds_u32*)((lds_u8*)(uintptr_t)(4 * x) + kLdsByteHist * 4), 1u);
+      if (x != ((1u << (t & 31)) & 0xFFu)) lds_add((lds_u32*)((lds_u8*)(uintptr_t)(4 * x) + kLdsByteHist * 4), 1u);
     }
+    if (lm >= D) lm -= D;
     if (prev0 == 0 && (q.x | q.y | q.z | q.w) != 0) {
       const uint32_t vfirst = chunk_first(q, boff);
       lds_add((lds_u32*)(H8 + 2 * min(8u * vfirst - vb, kCap8)), 0xFFFFFFFFu);
