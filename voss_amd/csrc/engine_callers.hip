@@ -43,8 +43,13 @@ std::vector<uint64_t> Engine::goldbach_upto(uint64_t N, uint64_t max_ce, uint64_
   uint64_t lim = std::min<uint64_t>(n_top / 2, plim);
   if (scan_limit) lim = std::min(lim, scan_limit);
   lim = std::min<uint64_t>(lim, table_limit_);
-  std::vector<uint32_t> sp = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59};  // the table starts at 61
-  while (!sp.empty() && (uint64_t)sp.back() > lim) sp.pop_back();
+  // the odd primes below the sieving table's first prime, then the table
+  std::vector<uint32_t> sp;
+  for (uint32_t q = 3; q < kFirstSievingPrime && q <= lim; q += 2) {
+    bool pr = true;
+    for (uint32_t d = 3; d * d <= q; d += 2) pr = pr && (q % d != 0);
+    if (pr) sp.push_back(q);
+  }
   for (uint32_t p : h_primes_) {
     if ((uint64_t)p > lim) break;
     sp.push_back(p);

@@ -323,11 +323,11 @@ __device__ void cache_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   }
 }
 
-// The read patterns (multiples of 23..59 cleared) over one segment: the 16
+// The read patterns (multiples of 23..71 cleared) over one segment: the 16
 // bytes that AND into the segment's chunk at byte boff (a multiple of 16),
 // each from the copy whose phase makes the source word-aligned.  In a
 // segment starting below global byte 4, chunk 0 keeps its first 4 - k0
-// bytes as they are: init_segment wrote the true primes there (23..59 among
+// bytes as they are: init_segment wrote the true primes there (23..71 among
 // them).
 constexpr int kReadPatterns = kPresieveGroups - 1;
 struct ReadPatterns {
@@ -623,13 +623,36 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
     else if (it >= e1 && it < e3) { i = n2 + 64 * (it - e1) + lane; end = n3; }
     return i < end ? A.primes[i] : 0u;
   };
-  uint32_t it = take();
+  // Taking order: one tier-2 chunk per wave first, then the other tier-2
+  // chunks spread evenly among the tier-0/1 items (position j of that run is
+  // a chunk when floor((j+1) Am / Mm) > floor(j Am / Mm)), then tier 3, so
+  // waves on the setup-bound one-prime items overlap waves on LDS-bound
+  // chunks.  remap turns the counter's value into the item index above.
+  const uint32_t H = min(e2, (uint32_t)NW);
+  const uint32_t Am = e2 - H, Mm = e1 - H;
+  // (the float quotient below is exact to within one for products below
+  // 2^24, far above any tier layout the engine configures)
+  const bool spread = Am != 0 && (uint64_t)Am * Mm < (1u << 24);
+  auto remap = [&](uint32_t t) -> uint32_t {
+    if (t < H || t >= e1 || !spread) return t;
+    const uint32_t j = t - H;
+    auto fl = [&](uint32_t x) -> uint32_t {  // floor(x * Am / Mm), exact
+      const uint32_t y = x * Am;
+      uint32_t q = (uint32_t)((float)y * __builtin_amdgcn_rcpf((float)Mm));
+      if (q * Mm > y) --q;
+      if ((q + 1) * Mm <= y) ++q;
+      return q;
+    };
+    const uint32_t a0 = fl(j), a1 = fl(j + 1);
+    return a1 > a0 ? H + a0 : e2 + (j - a0);
+  };
+  uint32_t it = remap(take());
   uint32_t e = lane_entry(it);
   // the first entry is waited for here, so that inside the loop only the
   // next item's entry is in flight and its wait falls at the loop's end
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   while (it < e3) {
-    const uint32_t nit = take();
+    const uint32_t nit = remap(take());
     const uint32_t ne = lane_entry(nit);
     const uint32_t p = e & 0xFFFFFFu, r = e >> 24;
     if (it < e2) {
