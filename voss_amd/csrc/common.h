@@ -12,15 +12,19 @@ namespace voss {
 
 // Primes 7..19 are removed by copying a periodic byte pattern (period
 // 7*11*13*17*19 = 323,323 bytes) into LDS as the segment's initial bytes;
-// 23..71 by AND-ing four more patterns (periods 23*29*31 = 20,677,
-// 37*41*43 = 65,231, 47*53*59 = 147,029 and 61*67*71 = 290,177 bytes; the
+// 23..109 by AND-ing seven more patterns, three consecutive primes each
+// (periods 23*29*31 = 20,677 bytes up to 103*107*109 = 1,201,289; the
 // "read patterns") into every 16-byte chunk where the sieved segment is
 // read: crossing-off never reads the bitmap, so the AND may come last.
-// Crossing-off starts at 73, the sieving table's first prime.
-constexpr int kPresieveGroups = 5;
-constexpr uint32_t kPresievePeriods[kPresieveGroups] = {7u * 11u * 13u * 17u * 19u, 23u * 29u * 31u, 37u * 41u * 43u,
-                                                        47u * 53u * 59u, 61u * 67u * 71u};
-constexpr uint32_t kFirstSievingPrime = 73;
+// Crossing-off starts at 113, the sieving table's first prime.
+constexpr int kPresieveGroups = 8;
+constexpr uint32_t kPresievePeriods[kPresieveGroups] = {
+    7u * 11u * 13u * 17u * 19u, 23u * 29u * 31u, 37u * 41u * 43u, 47u * 53u * 59u,
+    61u * 67u * 71u,            73u * 79u * 83u, 89u * 97u * 101u, 103u * 107u * 109u};
+constexpr uint32_t kFirstSievingPrime = 113;
+// every presieved prime lies in the bitmap's bytes 0..3 (values below 120),
+// which segment 0 keeps as their true bits
+static_assert(kFirstSievingPrime <= 120, "presieved primes must lie below 120");
 // Primes below this are sieved by tier 0 (a half-wave per prime and class).
 constexpr uint32_t kTier0P = 192;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment

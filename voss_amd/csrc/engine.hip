@@ -135,7 +135,8 @@ Engine::Engine() {
   // full segment so a segment never wraps.  Pattern 0 is a segment's initial
   // bytes, the others (the read patterns) are ANDed in as it is read.
   static const uint32_t groups[kPresieveGroups][5] = {
-      {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0}, {37, 41, 43, 0, 0}, {47, 53, 59, 0, 0}, {61, 67, 71, 0, 0}};
+      {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0},  {37, 41, 43, 0, 0},   {47, 53, 59, 0, 0},
+      {61, 67, 71, 0, 0},  {73, 79, 83, 0, 0},  {89, 97, 101, 0, 0}, {103, 107, 109, 0, 0}};
   std::vector<uint32_t> words;
   for (int gi = 0; gi < kPresieveGroups; ++gi) {
     presieve_off_[gi] = (uint32_t)words.size();

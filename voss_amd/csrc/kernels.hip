@@ -323,11 +323,11 @@ __device__ void cache_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   }
 }
 
-// The read patterns (multiples of 23..71 cleared) over one segment: the 16
+// The read patterns (multiples of 23..109 cleared) over one segment: the 16
 // bytes that AND into the segment's chunk at byte boff (a multiple of 16),
 // each from the copy whose phase makes the source word-aligned.  In a
 // segment starting below global byte 4, chunk 0 keeps its first 4 - k0
-// bytes as they are: init_segment wrote the true primes there (23..71 among
+// bytes as they are: init_segment wrote the true primes there (23..109 among
 // them).
 constexpr int kReadPatterns = kPresieveGroups - 1;
 struct ReadPatterns {
