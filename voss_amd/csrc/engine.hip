@@ -133,7 +133,9 @@ Engine::Engine() {
   // presieve patterns: in pattern g, byte j has bit o set iff 30j + W30[o] has
   // no factor among the group's primes; each is stored for one period plus a
   // full segment so a segment never wraps.  Pattern 0 is a segment's initial
-  // bytes, the others (the read patterns) are ANDed in as it is read.
+  // bytes, the others (the read patterns) are ANDed in as it is read.  Each
+  // is built as a small sieve: the multiples of q in class o are the bytes
+  // j = j0 (mod q) with 30 j0 + W30[o] = 0 (mod q).
   static const uint32_t groups[kPresieveGroups][5] = {
       {7, 11, 13, 17, 19}, {23, 29, 31, 0, 0},  {37, 41, 43, 0, 0},   {47, 53, 59, 0, 0},
       {61, 67, 71, 0, 0},  {73, 79, 83, 0, 0},  {89, 97, 101, 0, 0}, {103, 107, 109, 0, 0}};
@@ -141,16 +143,15 @@ Engine::Engine() {
   for (int gi = 0; gi < kPresieveGroups; ++gi) {
     presieve_off_[gi] = (uint32_t)words.size();
     const size_t len = ((size_t)kPresievePeriods[gi] + kMaxSegBytes + 4096 + 3) & ~(size_t)3;
-    std::vector<uint8_t> pat(len + 4);
-    for (size_t j = 0; j < len + 4; ++j) {
-      uint8_t b = 0;
+    std::vector<uint8_t> pat(len + 4, 0xFF);
+    for (int qi = 0; qi < 5 && groups[gi][qi]; ++qi) {
+      const uint32_t q = groups[gi][qi];
       for (int o = 0; o < 8; ++o) {
-        const uint64_t v = 30ull * j + kW30[o];
-        bool keep = true;
-        for (int q = 0; q < 5 && groups[gi][q]; ++q) keep = keep && (v % groups[gi][q] != 0);
-        if (keep) b |= (uint8_t)(1u << o);
+        uint32_t j0 = 0;
+        while ((30ull * j0 + kW30[o]) % q != 0) ++j0;  // j0 < q
+        const uint8_t clear = (uint8_t)~(1u << o);
+        for (size_t j = j0; j < len + 4; j += q) pat[j] &= clear;
       }
-      pat[j] = b;
     }
     // four copies, copy c starting c bytes in, so a read at any byte phase
     // starts on a word of one of them (the kernel copies it to LDS directly)

@@ -997,6 +997,10 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
     // np = vb - 8 * (the last prime so far), or 0 before the wave's first
     uint32_t np = prev0 ? vb - 8u * prev0 : 0u, lm = 0;
     const uint32_t ws[4] = {q.x, q.y, q.z, q.w};
+    // bit-reversed words: byte 3 - k of rs[i] is byte k of ws[i] reversed, so
+    // its lowest set bit is 7 minus the highest set bit of byte k
+    const uint32_t rs[4] = {__builtin_bitreverse32(q.x), __builtin_bitreverse32(q.y), __builtin_bitreverse32(q.z),
+                            __builtin_bitreverse32(q.w)};
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const uint32_t x = (ws[j >> 2] >> (8 * (j & 3))) & 0xFFu;
@@ -1013,7 +1017,7 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
       lm = max(lm, g8);
       if (x) {
         lds_add((lds_u32*)(uintptr_t)(H8m + 2u * min(g8, kCap8 + D)), 1u);
-        const uint32_t pl = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, __builtin_clz(x << 24));
+        const uint32_t pl = __builtin_amdgcn_perm(kW30x8RevHi, kW30x8RevLo, ffbl_byte(rs[j >> 2], 3 - (j & 3)));
         np = (0xE8E8E800u - 240u * j) - pl;
       }
       // two or more set bits: x is not its lowest set bit (1 << t; 0 for x = 0)
