@@ -380,7 +380,7 @@ def main():
                      "traffic": traffic,
                      "algorithmic_ops_per_launch": algo_ops, "algorithmic_lds_bytes_per_launch": algo_bytes,
                      "init_and_count_bytes": init_count_bytes, "crossings": C,
-                     "note": "LDS-resident sieve: HBM carries only tables and per-segment summaries"},
+                     "note": "LDS-resident sieve: HBM carries the read patterns (up to 1.2 MB per segment, streamed from beyond the L2), tables and per-segment summaries"},
     }
     if not args.no_batch:
         line["batch"] = bench_batch(st, reps=5)
