@@ -26,6 +26,12 @@ __device__ __forceinline__ uint32_t w30(uint32_t o) {
   return __builtin_amdgcn_perm(kW30Hi, kW30Lo, o | 0x0C0C0C00u);
 }
 
+// A load that every lane makes from the same address of memory no kernel
+// writes (a scalar load through the constant address space).
+__device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p, uint32_t i) {
+  return ((const __attribute__((address_space(4))) uint32_t*)p)[i];
+}
+
 // Primality of v from a wheel bitmap whose byte 0 is k = 0 (covers up to limit).
 __device__ __forceinline__ bool bm_prime(const uint8_t* bm, uint64_t v) {
   if (v < 7) return v == 2 || v == 3 || v == 5;
@@ -214,27 +220,100 @@ __global__ void next_prev_kernel(const uint64_t* x, uint64_t n, uint64_t* out, i
   out[i] = res;
 }
 
-// Even n in [n_lo, n_hi] (both even, n_lo >= 6), a grid-stride loop over
-// them: search the ascending partners sp[] (odd primes, those <= n/2 only)
-// for one with n - p in the bitmap (goldbach_upto.cu:25-46) and append the n
-// that have none.  The slot counter keeps counting past cap, so the caller
-// sees how many there were.
-__global__ void goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t n_lo,
-                                      uint64_t n_hi, uint64_t* ce, unsigned long long* nce, uint64_t cap) {
-  const uint64_t count = (n_hi - n_lo) / 2 + 1;
-  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < count;
-       idx += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t n = n_lo + 2 * idx;
-    const uint64_t half = n / 2;
-    bool found = false;
-    for (uint32_t i = 0; i < nsp; ++i) {
-      const uint64_t p = sp[i];
-      if (p > half) break;
-      if (bm_prime(bm, n - p)) { found = true; break; }
+// The n (even, in [n_lo, n_hi], n_lo >= 6) with no partner among the
+// ascending odd primes sp[] (those <= n/2 only) such that n - p is prime
+// (goldbach_upto.cu:25-46), appended to ce; the slot counter keeps counting
+// past cap, so the caller sees how many there were.
+//
+// Fifteen n at a time: a thread takes the block of n = 30K + 2m, m = 0..14,
+// and tries the partners in ascending order for all fifteen at once.  For
+// p = 30a + r, n - p = 30(K - a) + 2m - r: bit o (residue w = W30[o]) of
+// byte K - a - b_o, where b_o = [w + r >= 30], decides n = 30K + 2m_o with
+// m_o = (w + r - 30 b_o) / 2, and the other seven m give n - p divisible by
+// 3 or 5.  So one partner resolves its n through two table lookups, the
+// bits of byte K - a and of byte K - a - 1 scattered to their m (a table per
+// residue r of p, built in LDS).  A wave stops when every lane's n are
+// resolved or its partners exceed n / 2.  Blocks K < 4 (n < 120, where n - p
+// may be 3 or 5, which the wheel bitmap does not hold) test one n at a time.
+constexpr uint32_t kGcThreads = 256;
+constexpr uint32_t kGcR = 10;  // residues of odd primes mod 30: 1, 3, 5, 7, 11, 13, 17, 19, 23, 29
+
+__device__ __forceinline__ uint32_t gc_rindex(uint32_t r) {
+  // 1->0 3->1 5->2 7->3 11->4 13->5 17->6 19->7 23->8 29->9
+  return r == 1 ? 0u : r == 3 ? 1u : r == 5 ? 2u : r == 7 ? 3u : r == 11 ? 4u : r == 13 ? 5u
+       : r == 17 ? 6u : r == 19 ? 7u : r == 23 ? 8u : 9u;
+}
+
+__global__ void __launch_bounds__(kGcThreads) goldbach_check_kernel(const uint8_t* bm, const uint32_t* sp, uint32_t nsp,
+                                                                   uint64_t n_lo, uint64_t n_hi, uint64_t* ce,
+                                                                   unsigned long long* nce, uint64_t cap) {
+  // tab[b][ri][x]: the n (bits m = 0..14) that byte value x at K - a - b
+  // proves for partners of residue index ri
+  __shared__ uint16_t tab[2][kGcR][256];
+  constexpr uint32_t kR[kGcR] = {1, 3, 5, 7, 11, 13, 17, 19, 23, 29};
+  for (uint32_t e = threadIdx.x; e < 2 * kGcR * 256; e += kGcThreads) {
+    const uint32_t x = e & 255, ri = (e >> 8) % kGcR, b = e / (kGcR * 256);
+    const uint32_t r = kR[ri];
+    uint32_t mask = 0;
+    for (uint32_t o = 0; o < 8; ++o) {
+      const uint32_t s = w30(o) + r, bo = s >= 30 ? 1u : 0u;
+      if (bo == b && ((x >> o) & 1u)) mask |= 1u << ((s - 30 * bo) >> 1);
     }
-    if (found) continue;
-    const unsigned long long slot = atomicAdd(nce, 1ull);
-    if (slot < cap) ce[slot] = n;
+    tab[b][ri][x] = (uint16_t)mask;
+  }
+  __syncthreads();
+  const uint64_t k_lo = n_lo / 30, k_hi = n_hi / 30;
+  const uint64_t stride = (uint64_t)gridDim.x * kGcThreads;
+  // the loop bound is the same for every lane of a wave up to its last
+  // iteration, where some lanes leave; the ballots below count only the
+  // lanes still in it
+  for (uint64_t K = k_lo + (uint64_t)blockIdx.x * kGcThreads + threadIdx.x; K <= k_hi; K += stride) {
+    // the block's n inside [n_lo, n_hi]
+    uint32_t U = 0x7FFFu;
+    const uint64_t n0 = 30 * K;
+    if (n0 < n_lo) U &= 0x7FFFu << (uint32_t)((n_lo - n0) >> 1);
+    if (n0 + 28 > n_hi) U &= 0x7FFFu >> (uint32_t)((n0 + 28 - n_hi) >> 1);
+    if (K < 4) {
+      // small n: one at a time, with 3 and 5 as partners' partners
+      for (uint32_t m = 0; m < 15; ++m) {
+        if (!((U >> m) & 1u)) continue;
+        const uint64_t n = n0 + 2 * m, half = n / 2;
+        for (uint32_t i = 0; i < nsp; ++i) {
+          const uint64_t p = sp[i];
+          if (p > half) break;
+          if (bm_prime(bm, n - p)) { U &= ~(1u << m); break; }
+        }
+      }
+    } else {
+      // partners p <= 15K + 14 can serve some n of the block; p serves the
+      // n with m >= p - 15K.  Four partners a round: their byte loads are
+      // independent, so a round waits for one memory latency.
+      const uint64_t plast = 15 * K + 14;
+      for (uint32_t i = 0; i < nsp; i += 4) {
+        const bool live = U != 0 && (uint64_t)load_uniform_u32(sp, i) <= plast;
+        if (!__builtin_amdgcn_ballot_w64(live)) break;
+        uint32_t hit = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          if (i + u >= nsp) break;  // uniform
+          const uint32_t p = load_uniform_u32(sp, i + u);
+          if (live && (uint64_t)p <= plast) {
+            const uint32_t a = p / 30, ri = gc_rindex(p - 30 * a);
+            const uint64_t kb = K - a;  // >= K - (K / 2 + 1) >= 1 for K >= 4
+            const uint32_t h = (uint32_t)tab[0][ri][bm[kb]] | (uint32_t)tab[1][ri][bm[kb - 1]];
+            const uint64_t d = (uint64_t)p > 15 * K ? (uint64_t)p - 15 * K : 0;
+            hit |= h & (0x7FFFu << (uint32_t)d);
+          }
+        }
+        U &= ~hit;
+      }
+    }
+    while (U) {
+      const uint32_t m = __builtin_ctz(U);
+      U &= U - 1;
+      const unsigned long long slot = atomicAdd(nce, 1ull);
+      if (slot < cap) ce[slot] = n0 + 2 * m;
+    }
   }
 }
 
@@ -359,10 +438,10 @@ hipError_t launch_next_prev(const uint64_t* x, uint64_t n, uint64_t* out, int di
 hipError_t launch_goldbach_check(const uint8_t* bm, const uint32_t* sp, uint32_t nsp, uint64_t n_lo, uint64_t n_hi,
                                  uint64_t* ce, unsigned long long* nce, uint64_t cap, hipStream_t st) {
   if (n_lo < 6 || n_hi < n_lo || (n_lo & 1) || (n_hi & 1)) return hipErrorInvalidValue;
-  const uint64_t cnt = (n_hi - n_lo) / 2 + 1;
-  const uint64_t blocks = std::min<uint64_t>((cnt + 255) / 256, 1u << 20);  // the kernel strides over the rest
-  hipLaunchKernelGGL(goldbach_check_kernel, dim3((uint32_t)blocks), dim3(256), 0, st, bm, sp, nsp, n_lo, n_hi, ce,
-                     nce, cap);
+  const uint64_t nblk = n_hi / 30 - n_lo / 30 + 1;  // blocks of fifteen n
+  const uint64_t blocks = std::min<uint64_t>((nblk + kGcThreads - 1) / kGcThreads, 1u << 16);  // the kernel strides over the rest
+  hipLaunchKernelGGL(goldbach_check_kernel, dim3((uint32_t)blocks), dim3(kGcThreads), 0, st, bm, sp, nsp, n_lo, n_hi,
+                     ce, nce, cap);
   return hipGetLastError();
 }
 hipError_t launch_goldbach_count(const uint8_t* bm, const uint64_t* ns, uint64_t k, uint64_t n_max, uint64_t plim,
