@@ -130,7 +130,8 @@ def cpu_other_configs(st, gold):
     """The oracle on the host cores beside the GPU for the other configurations:
     C0 pi(1e9) whole; the 4096-range batch on its first 256 queries; the
     [2, 1e12] sweep on its top 2e10 integers (range statistics there: the same
-    sieve work per integer as the sweep's hardest stretch)."""
+    sieve work per integer as the sweep's hardest stretch); and the Goldbach
+    check and the Sophie Germain count to 1e8, each beside the GPU's time."""
     from oracle import oracle as O
     threads = os.cpu_count() or 1
     out = {}
@@ -156,6 +157,28 @@ def cpu_other_configs(st, gold):
     dt = time.perf_counter() - t0
     out["c4_sweep_sample"] = {"cpu_ints_per_s": (b - a) / dt, "cpu_s": dt, "cores": threads,
                               "sample": f"range statistics over [{a}, {b}] (the sweep's top 2%)"}
+    # the callers beside the sweep, GPU against the oracle on the same input
+    n = 10**8
+    t0 = time.perf_counter()
+    n_ce, _ = O.goldbach_test_upto(n, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    voss_amd.conjectures.test_goldbach_upto(n)  # warm
+    t0 = time.perf_counter()
+    g = voss_amd.conjectures.test_goldbach_upto(n)
+    gpu_s = time.perf_counter() - t0
+    out["goldbach_check_1e8"] = {"cpu_s": cpu_s, "gpu_s": gpu_s, "exact": n_ce == 0 and not g["counterexamples"],
+                                 "cores": threads, "sample": "every even n in [4, 1e8] has a partition"}
+    t0 = time.perf_counter()
+    sg = O.sophie_germain(n)
+    cpu_s = time.perf_counter() - t0
+    with voss_amd.primes.Context(n) as c:
+        c.sophie_germain()  # warm
+    t0 = time.perf_counter()
+    with voss_amd.primes.Context(n) as c:
+        gsg = c.sophie_germain()
+    gpu_s = time.perf_counter() - t0
+    out["sophie_germain_1e8"] = {"cpu_s": cpu_s, "gpu_s": gpu_s, "count": gsg, "exact": gsg == sg == 423_140,
+                                 "cores": 1, "sample": "primes p <= 1e8 with 2p + 1 prime (the oracle is single-threaded)"}
     return out
 
 
