@@ -572,6 +572,31 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
   }
 }
 
+// The crossings of a prime with MINK <= K <= MINK + 1 crossings in every
+// class (MINK p <= nbytes <= (MINK + 1) p, no p*p rule): MINK unconditional,
+// the last one predicated, no loop.
+template <int MINK>
+__device__ __forceinline__ void cross_prime_banded(uint32_t base, uint32_t p, uint32_t r, uint32_t rk,
+                                                   uint32_t nbytes, const lds_u8* tab) {
+  const uint32_t q30 = p / 30;
+  const uint64_t frac = *(const lds_u64*)(tab + r * 8);
+  const uint64_t bits = *(const lds_u64*)(tab + 64 + r * 8);
+  const uint32_t rp = rot_step(p);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t cm = __umul24(q30, w30_const(c)) + ((uint32_t)((frac >> (8 * c)) & 0xFF) - rk);
+    uint32_t off = min(cm, cm + p);
+    uint32_t m = byte_mask(row_mask(bits, c), off);
+#pragma unroll
+    for (int k = 0; k < MINK; ++k) {
+      lds_and_at(base + (off & ~3u), m);
+      off += p;
+      m = rotate_by(m, rp);
+    }
+    if (off < nbytes) lds_and_at(base + (off & ~3u), m);
+  }
+}
+
 // Cross off every sieving prime >= 23 with p*p <= the segment's top value.
 //   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes splitting
 //     the class's crossings into runs whose LDS banks never collide;
@@ -580,8 +605,11 @@ __device__ __forceinline__ void cross_prime_classes(uint32_t base, uint32_t p, u
 //   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
 //     unrolled loop;
 //   tier 3 (seg_bytes/4 <= p < seg_bytes): one prime per lane, at most four
-//     crossings per class, so the per-class setup is kept to a few
-//     instructions (cross_prime_classes);
+//     crossings per class: where every lane's prime has K or K + 1 in each
+//     class (a band between nbytes/(K+1) and nbytes/K, which a chunk of
+//     consecutive primes nearly always lies in) K unconditional crossings and
+//     one predicated (cross_prime_banded), else a loop per class
+//     (cross_prime_classes);
 //   tier 4 (p >= seg_bytes): one prime per lane, all eight classes checked
 //     for their single possible crossing (cross_classes_once).
 template <int BS>
@@ -699,11 +727,15 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
         if (j < hi) cross_class(base, j, p1, hi, mask_word(tab[64 + r1 * 8 + c]));
       }
     } else {
-      // tier 3: one prime per lane, at most four crossings per class
+      // tier 3: one prime per lane, at most four crossings per class; the
+      // wave takes a band's loop-free form when all its primes lie in it
       if (e) {
         const uint32_t rk = mod_u64(k0, p);
         if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<false, true>(base, p, r, rk, k0, nbytes, tab, plow);
         else if (__builtin_amdgcn_ballot_w64(p >= nbytes)) cross_prime_classes<false, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else if (!__builtin_amdgcn_ballot_w64(3 * p > nbytes || 4 * p < nbytes)) cross_prime_banded<3>(base, p, r, rk, nbytes, tab);
+        else if (!__builtin_amdgcn_ballot_w64(2 * p > nbytes || 3 * p < nbytes)) cross_prime_banded<2>(base, p, r, rk, nbytes, tab);
+        else if (!__builtin_amdgcn_ballot_w64(2 * p < nbytes)) cross_prime_banded<1>(base, p, r, rk, nbytes, tab);
         else cross_prime_classes<false, false, true>(base, p, r, rk, k0, nbytes, tab, plow);
       }
     }
@@ -919,7 +951,8 @@ __device__ void chunk_big_gaps(const SieveArgs& A, uint32_t* lds, uint4 q, uint3
 // prime keeps the largest such distance instead, which never exceeds the
 // gap that closes over it, so `lm` stays a bound.
 template <int MODE, int BS>
-__device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split) {
+__device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g, uint32_t split,
+                             const PatternWords& bfirst) {
   constexpr int NW = BS / 64;
   const uint32_t lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
@@ -947,13 +980,11 @@ __device__ void walk_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& g
   const uint32_t wend = min(wbeg + wave_bytes, g.nbytes);
   // the read patterns are ANDed in as each chunk is read (not for batch
   // segments filled from the resident cache, whose bytes are final); the
-  // next step's are loaded one step ahead
+  // next step's are loaded one step ahead, the first step's (bfirst) by the
+  // caller before the barrier that precedes the walk
   const bool apply_rp = !(MODE == kModeBatch && g.from_cache);  // uniform per workgroup
   const ReadPatterns RP = read_patterns(A, g.k0);
-  PatternWords bnext;
-#pragma unroll
-  for (int i = 0; i < kReadPatterns; ++i) bnext.w[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
-  if (apply_rp && wbeg < wend) bnext = load_pattern_words(RP, wbeg + 16 * lane);
+  PatternWords bnext = bfirst;
   // waves further along drop in issue priority, so lagging waves catch up
   // and the walk's waves reach the barrier together
   __builtin_amdgcn_s_setprio(3);
@@ -1188,6 +1219,16 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
     if ((MODE == kModeStats || MODE == kModeCount) && threadIdx.x == 64)
       lds[kLdsMisc + kMiscItem] = gridDim.x + atomicAdd(A.work, 1u);
     if (!cached) sieve_segment<BS>(A, lds, g);
+    // the walk's first read-pattern words (all ones where none apply), loaded
+    // before the barrier so that their latency overlaps the wait for the
+    // other waves (the barrier waits for LDS operations only)
+    PatternWords bfirst;
+#pragma unroll
+    for (int i = 0; i < kReadPatterns; ++i) bfirst.w[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    if (MODE != kModeBitmap && !cached) {
+      const uint32_t wbeg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * (A.seg_bytes / NW);
+      if (wbeg < g.nbytes) bfirst = load_pattern_words(read_patterns(A, g.k0), wbeg + 16 * (threadIdx.x & 63));
+    }
     __syncthreads();
     if (MODE == kModeBitmap) {
       const uint8_t* seg8 = reinterpret_cast<const uint8_t*>(lds + kLdsSeg);
@@ -1212,7 +1253,7 @@ __global__ void __launch_bounds__(BS) sieve_kernel(SieveArgs A) {
       b0 = lds[kLdsMisc + kMiscSplit + 1];
       b1 = lds[kLdsMisc + kMiscSplit + 2];
     }
-    walk_segment<MODE, BS>(A, lds, g, split);
+    walk_segment<MODE, BS>(A, lds, g, split, bfirst);
     __syncthreads();
 
     if (threadIdx.x < 64) {
