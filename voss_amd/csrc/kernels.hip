@@ -423,15 +423,6 @@ __device__ __forceinline__ uint32_t row_mask(uint64_t row, int c) {
 }
 __device__ __forceinline__ uint32_t mask_word(uint32_t nb) { return 0xFFFFFF00u | nb; }
 
-// x mod p for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
-__device__ __forceinline__ uint32_t mod_u32(uint32_t x, uint32_t p) {
-  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)p));
-  int32_t r = (int32_t)(x - q * p);
-  if (r < 0) r += (int32_t)p;
-  if (r >= (int32_t)p) r -= (int32_t)p;
-  return (uint32_t)r;
-}
-
 // Clear the class bit (mask word mc) of bytes j, j+p, j+2p, ... below `end`
 // (segment byte offsets; base = LDS byte address of the segment).  The
 // byte-in-dword pattern repeats every 4 crossings, so the four dword deltas
@@ -462,27 +453,50 @@ __device__ __forceinline__ void cross_class(uint32_t base, uint32_t j, uint32_t 
   if (jt + 2 * p < end) lds_and_at(a + d2, m2);
 }
 
-// Clear the class bit (mask word mc) of cnt bytes s, s+p, s+2p, ... (as
-// cross_class, counted).
-__device__ __forceinline__ void cross_run(uint32_t base, uint32_t s, uint32_t p, uint32_t cnt, uint32_t mc) {
+// Clear the class bit (mask word mc) of the crossings s + (128 r + t) p,
+// t = 0..3, r = 0, 1, ..., the first n of them: groups of four consecutive
+// crossings, 128 crossings apart (a half-wave's 32 lanes take the 32 groups
+// between).  The byte-in-dword pattern of a group is the same in every
+// round, so its deltas and masks are loop invariants.
+__device__ __forceinline__ void cross_rounds(uint32_t base, uint32_t s, uint32_t p, int32_t n, uint32_t mc) {
   const uint32_t j0 = s & 3;
   const uint32_t d1 = (j0 + p) & ~3u, d2 = (j0 + 2 * p) & ~3u, d3 = (j0 + 3 * p) & ~3u;
   const uint32_t rp = rot_step(p);
   const uint32_t m0 = byte_mask(mc, j0);
   const uint32_t m1 = rotate_by(m0, rp), m2 = rotate_by(m1, rp), m3 = rotate_by(m2, rp);
   uint32_t a = base + (s & ~3u);
-  const uint32_t step = 4 * p;
-  for (uint32_t g = cnt >> 2; g; --g) {
+  const uint32_t step = 128 * p;
+  while (n >= 4) {
     lds_and_at(a, m0);
     lds_and_at(a + d1, m1);
     lds_and_at(a + d2, m2);
     lds_and_at(a + d3, m3);
     a += step;
+    n -= 128;
   }
-  const uint32_t t = cnt & 3;
-  if (t > 0) lds_and_at(a, m0);
-  if (t > 1) lds_and_at(a + d1, m1);
-  if (t > 2) lds_and_at(a + d2, m2);
+  if (n > 0) lds_and_at(a, m0);
+  if (n > 1) lds_and_at(a + d1, m1);
+  if (n > 2) lds_and_at(a + d2, m2);
+}
+
+// Clear the class bit (mask word mc) of bytes s, s+8p, s+16p, ... below end:
+// every eighth crossing of a class, so the byte within the dword, and with it
+// the mask, never changes.
+__device__ __forceinline__ void cross_stride8(uint32_t base, uint32_t s, uint32_t p, uint32_t end, uint32_t mc) {
+  const uint32_t m = byte_mask(mc, s);
+  uint32_t a = base + (s & ~3u);
+  const uint32_t step = 8 * p;
+  const int32_t lim = (int32_t)(base + end - (s & 3u));  // a crossing at dword a exists while a < lim
+  while ((int32_t)(a + 3 * step) < lim) {
+    lds_and_at(a, m);
+    lds_and_at(a + step, m);
+    lds_and_at(a + 2 * step, m);
+    lds_and_at(a + 3 * step, m);
+    a += 4 * step;
+  }
+  if ((int32_t)a < lim) lds_and_at(a, m);
+  if ((int32_t)(a + step) < lim) lds_and_at(a + step, m);
+  if ((int32_t)(a + 2 * step) < lim) lds_and_at(a + 2 * step, m);
 }
 
 // floor(x / p) for x < 2^22, 0 < p < 2^24 (float reciprocal, one correction each way).
@@ -601,7 +615,8 @@ __device__ __forceinline__ void cross_prime_banded(uint32_t base, uint32_t p, ui
 //   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes splitting
 //     the class's crossings into runs whose LDS banks never collide;
 //   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
-//     lanes are 8 classes x 8 sub-ranges of the segment;
+//     lanes are 8 classes x 8 phases, lane (c, t) taking every eighth
+//     crossing of class c from the t-th, so its mask never changes;
 //   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
 //     unrolled loop;
 //   tier 3 (seg_bytes/4 <= p < seg_bytes): one prime per lane, at most four
@@ -693,9 +708,10 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       }
     } else if (it < e0) {
       // tier 0: item = (prime, class pair); half-wave h takes class 2q + h.
-      // The class's K crossings are split into runs of 4m (m odd, 32 runs
-      // cover K), so lane l starts l*m*p dwords after lane 0: with m*p odd,
-      // the 32 lanes of a half-wave always address 32 different LDS banks.
+      // The class's K crossings go in rounds of 128: lane l takes crossings
+      // 128 r + 4 l .. 128 r + 4 l + 3, so in each instruction lane l is l*p
+      // dwords after lane 0: p being odd, the 32 lanes of a half-wave always
+      // address 32 different LDS banks.
       const uint32_t t0 = it - e2;
       const uint32_t hl = lane & 31, half = lane >> 5;
       const uint32_t c = ((t0 & 3) << 1) | half;
@@ -705,27 +721,16 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       if (offl < nbytes) {
         const uint32_t j = (uint32_t)offl;
         const uint32_t K = div_u32(nbytes - 1 - j, p0) + 1;  // crossings j + k*p < nbytes
-        const uint32_t m = ((K + 127) >> 7) | 1u;
-        const uint32_t kb = hl * 4 * m;
-        if (kb < K) cross_run(base, j + kb * p0, p0, min(4 * m, K - kb), mask_word(tab[64 + r0 * 8 + c]));
+        if (4 * hl < K) cross_rounds(base, j + 4 * hl * p0, p0, (int32_t)(K - 4 * hl), mask_word(tab[64 + r0 * 8 + c]));
       }
     } else if (it < e1) {
-      // tier 1: one prime per wave, lanes = 8 classes x 8 sub-ranges
+      // tier 1: one prime per wave, lanes = 8 classes x 8 phases: lane
+      // (c, t) takes crossings t, t + 8, t + 16, ... of class c
       const uint32_t c = lane & 7, sr = lane >> 3;
-      const uint32_t sub_len = ((nbytes + 7) / 8 + 3) & ~3u;
-      const uint32_t lo = min(sr * sub_len, nbytes);
-      const uint32_t hi = min(lo + sub_len, nbytes);
       const uint32_t e1i = load_uniform(A.primes, n0 + (it - e0));
       const uint32_t p1 = e1i & 0xFFFFFFu, r1 = e1i >> 24;
-      const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab);
-      if (offl < hi) {
-        uint32_t j = (uint32_t)offl;
-        if (j < lo) {
-          const uint32_t back = mod_u32(lo - j, p1);
-          j = lo + (back ? p1 - back : 0u);
-        }
-        if (j < hi) cross_class(base, j, p1, hi, mask_word(tab[64 + r1 * 8 + c]));
-      }
+      const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab) + (uint64_t)sr * p1;
+      if (offl < nbytes) cross_stride8(base, (uint32_t)offl, p1, nbytes, mask_word(tab[64 + r1 * 8 + c]));
     } else {
       // tier 3: one prime per lane, at most four crossings per class; the
       // wave takes a band's loop-free form when all its primes lie in it
