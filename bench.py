@@ -45,6 +45,7 @@ sys.path.insert(0, str(ROOT))
 A100_INTS_PER_S_AT_1E11 = 1e11 / 5.36  # reference README.md:79 (A100), the published headline
 LDS_PEAK_BYTES = 256 * 128 * 2.4e9     # gfx950: 256 CUs x 128 B/clk/CU of LDS x 2.4 GHz
 PI_1E11 = 4_118_054_813                # A006880
+FIRST_SIEVING_PRIME = 113              # the kernel's first crossed-off prime (7..109 go by pattern)
 
 
 def small_primes_upto(x: int):
@@ -58,8 +59,8 @@ def small_primes_upto(x: int):
     return [int(v) for v in np.nonzero(flags)[0]]
 
 
-def crossings(N: int, primes_upto=small_primes_upto) -> int:
-    """C(N) = sum over primes 7 <= p <= sqrt(N) of #{m in [p, N/p] : gcd(m, 30) = 1}."""
+def crossings(N: int, primes_upto=small_primes_upto, p_min: int = 7) -> int:
+    """C(N) = sum over primes p_min <= p <= sqrt(N) (p >= 7) of #{m in [p, N/p] : gcd(m, 30) = 1}."""
     w = (1, 7, 11, 13, 17, 19, 23, 29)
 
     def coprime30_upto(x):
@@ -68,7 +69,7 @@ def crossings(N: int, primes_upto=small_primes_upto) -> int:
     r = math.isqrt(N)
     total = 0
     for p in primes_upto(r):
-        if p < 7:
+        if p < max(7, p_min):
             continue
         total += coprime30_upto(N // p) - coprime30_upto(p - 1)
     return total
@@ -365,6 +366,9 @@ def main():
         exact = exact and r["prime_count"] == PI_1E11
 
     C = crossings(N)
+    # the kernel removes the multiples of 7..109 by pattern (init copy and
+    # read patterns) and crosses off from 113: the LDS atomics it makes
+    C_made = crossings(N, p_min=FIRST_SIEVING_PRIME)
     sieve_ms = sum(k["sieve_ms"] for k in kernel_ms) / len(kernel_ms)
     algo_ops = C + r["prime_count"]
     init_count_bytes = 2 * (N // 30 + 1)  # each bitmap byte written once (init) and read once (walk)
@@ -403,6 +407,8 @@ def main():
                      "traffic": traffic,
                      "algorithmic_ops_per_launch": algo_ops, "algorithmic_lds_bytes_per_launch": algo_bytes,
                      "init_and_count_bytes": init_count_bytes, "crossings": C,
+                     "crossings_made": C_made,
+                     "frac_made": (4 * (C_made + r["prime_count"]) + init_count_bytes) / (sieve_ms / 1e3) / LDS_PEAK_BYTES,
                      "note": "LDS-resident sieve: HBM carries the read patterns (up to 1.2 MB per segment, streamed from beyond the L2), tables and per-segment summaries"},
     }
     if not args.no_batch:
