@@ -241,7 +241,7 @@ class Engine {
   uint64_t table_limit_ = 0;  // tables cover sieving primes <= table_limit_
   uint32_t tier0_p_ = kTier0P;  // primes below: kernel tier 0
   uint32_t presieve_copy_words_[kPresieveGroups] = {0};
-  uint32_t tier2_div_ = 4;      // primes below seg_bytes / this: kernel tier 2
+  uint32_t tier2_div_ = 5;      // primes below seg_bytes / this: kernel tier 2
   KernelTimes times_;
   // device address of the no-allocation scratch array (null if the lookup
   // at start-up failed; scratch_err_ says why)

@@ -118,7 +118,7 @@ Engine::Engine() {
 
   load_tune_file();
   tier0_p_ = env_u32("VOSS_TIER0_P", kTier0P);
-  tier2_div_ = std::max<uint32_t>(1, env_u32("VOSS_TIER2_DIV", 4));
+  tier2_div_ = std::max<uint32_t>(1, env_u32("VOSS_TIER2_DIV", 5));
   if (std::getenv("VOSS_SEG_BYTES") || std::getenv("VOSS_BLOCK") || std::getenv("VOSS_WGS_PER_CU") ||
       std::getenv("VOSS_TIER1_P")) {
     TuneConfig c = cfgs_[0];

@@ -617,9 +617,9 @@ __device__ __forceinline__ void cross_prime_banded(uint32_t base, uint32_t p, ui
 //   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
 //     lanes are 8 classes x 8 phases, lane (c, t) taking every eighth
 //     crossing of class c from the t-th, so its mask never changes;
-//   tier 2 (p < seg_bytes/4): one prime per lane, each class walked by the
+//   tier 2 (p < seg_bytes/5): one prime per lane, each class walked by the
 //     unrolled loop;
-//   tier 3 (seg_bytes/4 <= p < seg_bytes): one prime per lane, at most four
+//   tier 3 (seg_bytes/5 <= p < seg_bytes): one prime per lane, at most five
 //     crossings per class: where every lane's prime has K or K + 1 in each
 //     class (a band between nbytes/(K+1) and nbytes/K, which a chunk of
 //     consecutive primes nearly always lies in) K unconditional crossings and
@@ -732,12 +732,13 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
       const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab) + (uint64_t)sr * p1;
       if (offl < nbytes) cross_stride8(base, (uint32_t)offl, p1, nbytes, mask_word(tab[64 + r1 * 8 + c]));
     } else {
-      // tier 3: one prime per lane, at most four crossings per class; the
+      // tier 3: one prime per lane, at most five crossings per class; the
       // wave takes a band's loop-free form when all its primes lie in it
       if (e) {
         const uint32_t rk = mod_u64(k0, p);
         if (__builtin_amdgcn_ballot_w64(p >= plow)) cross_prime_classes<false, true>(base, p, r, rk, k0, nbytes, tab, plow);
         else if (__builtin_amdgcn_ballot_w64(p >= nbytes)) cross_prime_classes<false, false>(base, p, r, rk, k0, nbytes, tab, plow);
+        else if (!__builtin_amdgcn_ballot_w64(4 * p > nbytes || 5 * p < nbytes)) cross_prime_banded<4>(base, p, r, rk, nbytes, tab);
         else if (!__builtin_amdgcn_ballot_w64(3 * p > nbytes || 4 * p < nbytes)) cross_prime_banded<3>(base, p, r, rk, nbytes, tab);
         else if (!__builtin_amdgcn_ballot_w64(2 * p > nbytes || 3 * p < nbytes)) cross_prime_banded<2>(base, p, r, rk, nbytes, tab);
         else if (!__builtin_amdgcn_ballot_w64(2 * p < nbytes)) cross_prime_banded<1>(base, p, r, rk, nbytes, tab);
