@@ -41,10 +41,20 @@ uint32_t env_u32(const char* name, uint32_t dflt) {
   return (uint32_t)std::strtoul(v, nullptr, 0);
 }
 
+// Timing events come from a per-thread free list: creating and destroying
+// four per call cost more than the calls' host work.
 struct Event {
   hipEvent_t e = nullptr;
-  Event() { hip_check(hipEventCreate(&e), "hipEventCreate"); }
-  ~Event() { if (e) (void)hipEventDestroy(e); }
+  static std::vector<hipEvent_t>& pool() {
+    static thread_local std::vector<hipEvent_t> p;
+    return p;
+  }
+  Event() {
+    auto& p = pool();
+    if (!p.empty()) { e = p.back(); p.pop_back(); }
+    else hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  ~Event() { if (e) pool().push_back(e); }
 };
 float elapsed(const Event& a, const Event& b) {
   float ms = 0;
