@@ -1475,16 +1475,33 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
       run = B;
     }
     if (S.maxgap > run) {
-      // list each wave range of the segment that raises the running maximum
-      const uint32_t* wg = A.wave_gaps + (size_t)s * A.waves;
-      for (uint32_t w = 0; w < A.waves; ++w) {
-        const uint32_t x = wg[w];
-        if ((x & 0xFFFFu) <= run) continue;
-        unsigned int slot = atomicAdd(A.n_rescan, 1u);
-        if (slot < A.rescan_cap) A.rescan[slot] = RescanItem{s, run, w, x >> 16};
-        run = x & 0xFFFFu;
+      // list each wave range of the segment that raises the running maximum:
+      // the segment's wave words in four 16-byte loads (one latency), the
+      // ranges counted, then one slot reservation for all of them
+      const uint4* wg = reinterpret_cast<const uint4*>(A.wave_gaps + (size_t)s * A.waves);
+      uint32_t x[16];
+#pragma unroll
+      for (uint32_t w4 = 0; w4 < 4; ++w4) {
+        const uint4 v = 4 * w4 < A.waves ? wg[w4] : make_uint4(0u, 0u, 0u, 0u);
+        x[4 * w4] = v.x; x[4 * w4 + 1] = v.y; x[4 * w4 + 2] = v.z; x[4 * w4 + 3] = v.w;
       }
-      run = max(run, S.maxgap);
+      uint32_t r = run, n = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < 16; ++w)
+        if ((x[w] & 0xFFFFu) > r) { ++n; r = x[w] & 0xFFFFu; }
+      if (n) {
+        const unsigned int slot = atomicAdd(A.n_rescan, n);
+        uint32_t i = 0;
+        r = run;
+#pragma unroll
+        for (uint32_t w = 0; w < 16; ++w)
+          if ((x[w] & 0xFFFFu) > r) {
+            if (slot + i < A.rescan_cap) A.rescan[slot + i] = RescanItem{s, r, w, x[w] >> 16};
+            ++i;
+            r = x[w] & 0xFFFFu;
+          }
+      }
+      run = max(r, S.maxgap);
     }
     prev = absval(s, S.last);
   });
