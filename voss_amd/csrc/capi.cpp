@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -593,24 +594,24 @@ extern "C" int voss_amd_full_stats(uint64_t N, uint64_t cp_interval, uint32_t la
   });
 }
 
-static void copy_range(const voss::RangeOut& r, voss_amd_range_t* o) {
-  o->count = r.count;
-  o->first = r.first;
-  o->last = r.last;
-  std::memcpy(o->hist, r.hist, sizeof(o->hist));
-  o->pad = 0;
-}
+// batch results are written by the engine in place: the two layouts agree
+static_assert(sizeof(voss::RangeOut) == sizeof(voss_amd_range_t), "RangeOut layout");
+static_assert(offsetof(voss::RangeOut, count) == offsetof(voss_amd_range_t, count) &&
+                  offsetof(voss::RangeOut, first) == offsetof(voss_amd_range_t, first) &&
+                  offsetof(voss::RangeOut, last) == offsetof(voss_amd_range_t, last) &&
+                  offsetof(voss::RangeOut, hist) == offsetof(voss_amd_range_t, hist) &&
+                  offsetof(voss::RangeOut, pad) == offsetof(voss_amd_range_t, pad),
+              "RangeOut layout");
 
 extern "C" int voss_amd_batch_ranges(const uint64_t* a, const uint64_t* b, uint64_t n, voss_amd_range_t* out) {
   if ((!a || !b || !out) && n) { set_err("null pointer"); return VOSS_ERR_INVALID_ARG; }
   for (uint64_t i = 0; i < n; ++i)
     if (b[i] > kMaxN) { set_err("b exceeds maximum (10^14)"); return VOSS_ERR_OUT_OF_RANGE; }
   return guarded([&] {
-    std::vector<voss::RangeOut> r(n);
+    // the engine writes each record once, straight into out (same layout)
     Engine& e = eng();
     std::lock_guard<std::mutex> lk(e.mu);
-    e.batch_ranges(a, b, n, r.data());
-    for (uint64_t i = 0; i < n; ++i) copy_range(r[i], out + i);
+    e.batch_ranges(a, b, n, reinterpret_cast<voss::RangeOut*>(out));
     return VOSS_OK;
   });
 }

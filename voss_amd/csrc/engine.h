@@ -50,9 +50,12 @@ struct SweepStats {
   uint64_t large_total = 0;                  // found (may exceed the stored list)
 };
 
+// Laid out as the C ABI's voss_amd_range_t (pad included), so batch results
+// can be written straight into a caller's array of those.
 struct RangeOut {
   uint64_t count = 0, first = 0, last = 0;
   uint32_t hist[kQueryBins] = {0};
+  uint32_t pad = 0;
 };
 
 struct KernelTimes {

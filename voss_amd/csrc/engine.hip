@@ -873,7 +873,11 @@ void Engine::stage_batch(BatchPlan& P, const BatchLayout& L, const uint64_t* a, 
 
 void Engine::read_batch(const BatchPlan* P, const BatchLayout& L, const uint64_t* a, const uint64_t* b, size_t n,
                         RangeOut* out) {
-  for (size_t i = 0; i < n; ++i) out[i] = RangeOut();
+  // every record written once (out may be uninitialised): zeros for the
+  // queries with no wheel bytes (those layout_batch leaves out), the device
+  // results for the others
+  for (size_t i = 0; i < n; ++i)
+    if (b[i] < std::max<uint64_t>(a[i], 7)) out[i] = RangeOut();
   if (L.nq) {
     const unsigned long long* qc = static_cast<const unsigned long long*>(P->h_out);
     const uint32_t* qh = reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(P->h_out) + (size_t)P->qcap * 24);
@@ -883,6 +887,7 @@ void Engine::read_batch(const BatchPlan* P, const BatchLayout& L, const uint64_t
       o.first = qc[P->qcap + q];
       o.last = qc[2 * P->qcap + q];
       std::memcpy(o.hist, qh + (size_t)q * kQueryBins, kQueryBins * 4);
+      o.pad = 0;
     }
   }
   // seeds 2, 3, 5 (not in the wheel bitmap) for queries that include them

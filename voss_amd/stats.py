@@ -105,13 +105,17 @@ def batch_ranges(a, b, as_numpy: bool = False):
     n = len(a)
     if len(b) != n:
         raise ValueError("a and b must have the same length")
-    out = (_capi.AmdRange * max(1, n))()
     ka, pa = _u64_arg(a)
     kb, pb = _u64_arg(b)
-    _capi._check(_L.voss_amd_batch_ranges(pa, pb, n, out), _ERRMAP)
     if as_numpy:
+        # the library writes every field of every record: no zero fill
         import numpy as np
-        return np.frombuffer(out, dtype=_range_dtype(), count=n)
+        arr = np.empty(max(1, n), dtype=_range_dtype())
+        _capi._check(_L.voss_amd_batch_ranges(pa, pb, n, arr.ctypes.data_as(ctypes.POINTER(_capi.AmdRange))),
+                     _ERRMAP)
+        return arr[:n]
+    out = (_capi.AmdRange * max(1, n))()
+    _capi._check(_L.voss_amd_batch_ranges(pa, pb, n, out), _ERRMAP)
     return [_range_dict(out[i]) for i in range(n)]
 
 
