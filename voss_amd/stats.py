@@ -37,8 +37,13 @@ def full_stats(N: int, cp_interval: int = 0, large_threshold: int = 500,
         hist = (ctypes.c_uint64 * 100000)() if with_hist else None
     ncp = N // cp_interval if cp_interval else 0
     cps = (ctypes.c_uint64 * max(1, 5 * ncp))()
-    lg = (_capi.AmdGap * large_cap)()
-    rec = (_capi.AmdGap * rec_cap)()
+    # the gap lists are read only up to the counts the library returns, so
+    # their buffers need no zero fill
+    import numpy as np
+    lg_buf = np.empty(max(1, large_cap) * ctypes.sizeof(_capi.AmdGap), dtype=np.uint8)
+    rec_buf = np.empty(max(1, rec_cap) * ctypes.sizeof(_capi.AmdGap), dtype=np.uint8)
+    lg = lg_buf.ctypes.data_as(ctypes.POINTER(_capi.AmdGap))
+    rec = rec_buf.ctypes.data_as(ctypes.POINTER(_capi.AmdGap))
     rc = _L.voss_amd_full_stats(N, cp_interval, large_threshold, ctypes.byref(st), hist,
                                 cps if ncp else None, ncp, lg, large_cap, rec, rec_cap)
     _capi._check(rc, _ERRMAP)
