@@ -46,3 +46,25 @@ def test_batch_shapes_replay(voss, oracle):
             for (a, b), g in zip(qs[:20], got[:20]):
                 o = oracle.range_stats(a, b)
                 assert (g["count"], g["first"], g["last"], g["hist"]) == (o["count"], o["first"], o["last"], o["hist"])
+
+
+def test_batch_numpy_records_fully_written(voss):
+    # the numpy path hands the library an uninitialised array: every field of
+    # every record, empty queries (b < a, no wheel byte) and the seeds 2, 3, 5
+    # included, must equal the zero-initialised list path; a freed array of
+    # the same size full of ones makes stale bytes likely
+    import numpy as np
+    qs = [(10, 5), (0, 1), (2, 2), (0, 6), (3, 3), (8, 10), (7, 7), (100, 130), (24, 28), (0, 0),
+          (10**12, 10**12 + 10**4), (5, 5), (1, 7)]
+    for rep in range(3):
+        junk = np.full(len(qs) * 1056, 0xFF, dtype=np.uint8)
+        del junk
+        arr = voss.stats.batch_ranges([a for a, _ in qs], [b for _, b in qs], as_numpy=True)
+        lst = voss.stats.batch_ranges([a for a, _ in qs], [b for _, b in qs])
+        assert len(arr) == len(lst) == len(qs)
+        for q, r, l in zip(qs, arr, lst):
+            assert (int(r["count"]), int(r["first"]), int(r["last"])) == (l["count"], l["first"], l["last"]), q
+            assert [int(h) for h in r["hist"]] == l["hist"], q
+            assert int(r["pad"]) == 0, q
+        qs = qs[::-1]
+    assert len(voss.stats.batch_ranges([], [], as_numpy=True)) == 0
