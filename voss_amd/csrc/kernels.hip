@@ -1420,8 +1420,14 @@ __global__ void __launch_bounds__(1024) stitch_kernel(StitchArgs A) {
   __shared__ uint32_t s_hist[kHistBins];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < kHistBins; i += 1024) s_hist[i] = 0;
-  const uint32_t per = (A.nseg + 1023) / 1024;
-  const uint32_t lo = min(t * per, A.nseg), hi = min(lo + per, A.nseg);
+  // records cluster in the first segments (maximal gaps grow slowly), and a
+  // segment that holds one costs a thread a chain of loads and atomics: the
+  // first 64 segments go one to a thread (threads 0-63), the rest evenly to
+  // threads 64-1023
+  const uint32_t early = min(A.nseg, 64u);
+  const uint32_t per = (A.nseg - early + 959) / 960;
+  const uint32_t lo = t < 64 ? min(t, early) : min(early + (t - 64) * per, A.nseg);
+  const uint32_t hi = t < 64 ? min(t + 1, early) : min(lo + per, A.nseg);
   auto absval = [&](uint32_t s, uint32_t rel) -> uint64_t {
     return 30ull * (A.k_base + (uint64_t)s * A.seg_bytes) + rel;
   };
