@@ -28,7 +28,7 @@ SHAPES = [  # (seg_bytes, block, wgs_per_cu)
     (147456, 1024, 1), (131072, 1024, 1), (114688, 1024, 1), (98304, 1024, 1),
     (65536, 1024, 2), (73728, 512, 1), (65536, 512, 2), (147456, 512, 1),
 ]
-TIER1 = [512, 1024, 2048, 4096, 8192]
+TIER1 = [768, 1024, 1536, 2048, 3072]
 
 
 def _lds_ok(seg, wgs):

@@ -29,7 +29,7 @@ struct TuneConfig {
   uint32_t seg_bytes = 144 * 1024;  // bytes of bitmap per segment (x30 integers)
   int block = 1024;                 // threads per workgroup
   int wgs_per_cu = 1;               // persistent workgroups per CU
-  uint32_t tier1_p = 2048;          // primes below this: one prime per wave
+  uint32_t tier1_p = 1536;          // primes below this: one prime per half-wave
 };
 
 struct GapEntry {

@@ -479,13 +479,15 @@ __device__ __forceinline__ void cross_rounds(uint32_t base, uint32_t s, uint32_t
   if (n > 2) lds_and_at(a + d2, m2);
 }
 
-// Clear the class bit (mask word mc) of bytes s, s+8p, s+16p, ... below end:
-// every eighth crossing of a class, so the byte within the dword, and with it
-// the mask, never changes.
-__device__ __forceinline__ void cross_stride8(uint32_t base, uint32_t s, uint32_t p, uint32_t end, uint32_t mc) {
+// Clear the class bit (mask word mc) of bytes s, s + PH p, s + 2 PH p, ...
+// below end: every PH-th crossing of a class (PH a multiple of 4), so the
+// byte within the dword, and with it the mask, never changes.
+template <uint32_t PH>
+__device__ __forceinline__ void cross_phase(uint32_t base, uint32_t s, uint32_t p, uint32_t end, uint32_t mc) {
+  static_assert(PH % 4 == 0, "the stride must keep the byte within the dword");
   const uint32_t m = byte_mask(mc, s);
   uint32_t a = base + (s & ~3u);
-  const uint32_t step = 8 * p;
+  const uint32_t step = PH * p;
   const int32_t lim = (int32_t)(base + end - (s & 3u));  // a crossing at dword a exists while a < lim
   while ((int32_t)(a + 3 * step) < lim) {
     lds_and_at(a, m);
@@ -612,11 +614,12 @@ __device__ __forceinline__ void cross_prime_banded(uint32_t base, uint32_t p, ui
 }
 
 // Cross off every sieving prime >= 23 with p*p <= the segment's top value.
-//   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes splitting
-//     the class's crossings into runs whose LDS banks never collide;
-//   tier 1 (p below the tier1_end-th prime): one prime per wave turn, the 64
-//     lanes are 8 classes x 8 phases, lane (c, t) taking every eighth
-//     crossing of class c from the t-th, so its mask never changes;
+//   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes taking
+//     the class's crossings in rounds whose LDS banks never collide;
+//   tier 1 (p below the tier1_end-th prime): two primes per wave turn, one
+//     per half-wave, whose 32 lanes are 8 classes x 4 phases, lane (c, t)
+//     taking every fourth crossing of class c from the t-th, so its mask
+//     never changes;
 //   tier 2 (p < seg_bytes/5): one prime per lane, each class walked by the
 //     unrolled loop;
 //   tier 3 (seg_bytes/5 <= p < seg_bytes): one prime per lane, at most five
@@ -649,7 +652,7 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
   // reach tier 4 together.  Tier 4 has its own counter below.
   const uint32_t e2 = (n2 - n1 + 63) >> 6;
   const uint32_t e0 = e2 + 4 * n0;
-  const uint32_t e1 = e0 + (n1 - n0);
+  const uint32_t e1 = e0 + (n1 - n0 + 1) / 2;
   const uint32_t e3 = e1 + ((n3 - n2 + 63) >> 6);
   auto take = [&]() -> uint32_t {
     uint32_t v = 0;
@@ -724,13 +727,19 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
         if (4 * hl < K) cross_rounds(base, j + 4 * hl * p0, p0, (int32_t)(K - 4 * hl), mask_word(tab[64 + r0 * 8 + c]));
       }
     } else if (it < e1) {
-      // tier 1: one prime per wave, lanes = 8 classes x 8 phases: lane
-      // (c, t) takes crossings t, t + 8, t + 16, ... of class c
-      const uint32_t c = lane & 7, sr = lane >> 3;
-      const uint32_t e1i = load_uniform(A.primes, n0 + (it - e0));
-      const uint32_t p1 = e1i & 0xFFFFFFu, r1 = e1i >> 24;
-      const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab) + (uint64_t)sr * p1;
-      if (offl < nbytes) cross_stride8(base, (uint32_t)offl, p1, nbytes, mask_word(tab[64 + r1 * 8 + c]));
+      // tier 1: two primes per wave, one per half-wave, whose lanes are 8
+      // classes x 4 phases: lane (c, t) takes crossings t, t + 4, t + 8, ...
+      // of class c
+      const uint32_t c = lane & 7, sr = (lane >> 3) & 3, half = lane >> 5;
+      const uint32_t i1 = n0 + 2 * (it - e0);
+      const uint32_t ea = load_uniform(A.primes, i1);
+      const uint32_t eb = i1 + 1 < n1 ? load_uniform(A.primes, i1 + 1) : 0u;
+      const uint32_t e1i = half ? eb : ea;
+      if (e1i) {
+        const uint32_t p1 = e1i & 0xFFFFFFu, r1 = e1i >> 24;
+        const uint64_t offl = first_crossing(p1, r1, c, mod_u64(k0, p1), k0, tab) + (uint64_t)sr * p1;
+        if (offl < nbytes) cross_phase<4>(base, (uint32_t)offl, p1, nbytes, mask_word(tab[64 + r1 * 8 + c]));
+      }
     } else {
       // tier 3: one prime per lane, at most five crossings per class; the
       // wave takes a band's loop-free form when all its primes lie in it
