@@ -26,7 +26,7 @@ constexpr uint32_t kFirstSievingPrime = 113;
 // which segment 0 keeps as their true bits
 static_assert(kFirstSievingPrime <= 120, "presieved primes must lie below 120");
 // Primes below this are sieved by tier 0 (a half-wave per prime and class).
-constexpr uint32_t kTier0P = 192;
+constexpr uint32_t kTier0P = 160;
 constexpr uint32_t kMaxSegBytes = 144u * 1024u;                  // LDS budget per segment
 // LDS gap histogram: bins index gap/2 and cover every gap a batched query
 // bins (2..512); longer gaps fall into the last bin and take an exact slow

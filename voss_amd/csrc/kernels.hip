@@ -614,7 +614,7 @@ __device__ __forceinline__ void cross_prime_banded(uint32_t base, uint32_t p, ui
 }
 
 // Cross off every sieving prime >= 23 with p*p <= the segment's top value.
-//   tier 0 (p < 192): a half-wave per (prime, class), its 32 lanes taking
+//   tier 0 (p < 160): a half-wave per (prime, class), its 32 lanes taking
 //     the class's crossings in rounds whose LDS banks never collide;
 //   tier 1 (p below the tier1_end-th prime): two primes per wave turn, one
 //     per half-wave, whose 32 lanes are 8 classes x 4 phases, lane (c, t)
