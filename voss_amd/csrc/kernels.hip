@@ -646,12 +646,11 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
 
   // Work items of tiers 0-3, heaviest first: the 64-prime chunks of tier 2
   // (its first chunk alone is a large share of a wave's work), tier-0
-  // (prime, class pair) items, tier-1 primes, then the 64-prime chunks of
-  // tier 3.  Each wave takes its next item from an LDS counter (cleared by
+  // primes, tier-1 prime pairs, then the 64-prime chunks of tier 3.  Each wave takes its next item from an LDS counter (cleared by
   // init_segment), so waves that drew heavy items take fewer and all of them
   // reach tier 4 together.  Tier 4 has its own counter below.
   const uint32_t e2 = (n2 - n1 + 63) >> 6;
-  const uint32_t e0 = e2 + 4 * n0;
+  const uint32_t e0 = e2 + n0;
   const uint32_t e1 = e0 + (n1 - n0 + 1) / 2;
   const uint32_t e3 = e1 + ((n3 - n2 + 63) >> 6);
   auto take = [&]() -> uint32_t {
@@ -710,21 +709,25 @@ __device__ void sieve_segment(const SieveArgs& A, uint32_t* lds, const SegGeom& 
         else cross_prime_classes<true, false, true>(base, p, r, rk, k0, nbytes, tab, plow);
       }
     } else if (it < e0) {
-      // tier 0: item = (prime, class pair); half-wave h takes class 2q + h.
-      // The class's K crossings go in rounds of 128: lane l takes crossings
+      // tier 0: item = prime; half-wave h takes classes h, h + 2, h + 4, h + 6
+      // in turn (k0 mod p computed once).  A class's K crossings go in
+      // rounds of 128: lane l takes crossings
       // 128 r + 4 l .. 128 r + 4 l + 3, so in each instruction lane l is l*p
       // dwords after lane 0: p being odd, the 32 lanes of a half-wave always
       // address 32 different LDS banks.
       const uint32_t t0 = it - e2;
       const uint32_t hl = lane & 31, half = lane >> 5;
-      const uint32_t c = ((t0 & 3) << 1) | half;
-      const uint32_t e0i = load_uniform(A.primes, t0 >> 2);
+      const uint32_t e0i = load_uniform(A.primes, t0);
       const uint32_t p0 = e0i & 0xFFFFFFu, r0 = e0i >> 24;
-      const uint64_t offl = first_crossing(p0, r0, c, mod_u64(k0, p0), k0, tab);
-      if (offl < nbytes) {
-        const uint32_t j = (uint32_t)offl;
-        const uint32_t K = div_u32(nbytes - 1 - j, p0) + 1;  // crossings j + k*p < nbytes
-        if (4 * hl < K) cross_rounds(base, j + 4 * hl * p0, p0, (int32_t)(K - 4 * hl), mask_word(tab[64 + r0 * 8 + c]));
+      const uint32_t rk0 = mod_u64(k0, p0);
+      for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t c = (q << 1) | half;
+        const uint64_t offl = first_crossing(p0, r0, c, rk0, k0, tab);
+        if (offl < nbytes) {
+          const uint32_t j = (uint32_t)offl;
+          const uint32_t K = div_u32(nbytes - 1 - j, p0) + 1;  // crossings j + k*p < nbytes
+          if (4 * hl < K) cross_rounds(base, j + 4 * hl * p0, p0, (int32_t)(K - 4 * hl), mask_word(tab[64 + r0 * 8 + c]));
+        }
       }
     } else if (it < e1) {
       // tier 1: two primes per wave, one per half-wave, whose lanes are 8
